@@ -1,0 +1,72 @@
+"""Builds the native parts of dqsim_bosonic_amd in-tree.
+
+* ``libdqsim_gpu.so``: the HIP engine (csrc/dq_engine.hip) for gfx950, exporting
+  the C ABI of include/dqsim_gpu.h.
+* ``_core*.so``: the C++ host (pybind11) that mirrors the reference's Python API
+  and calls the engine through that C ABI.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+ARCH = os.environ.get("DQ_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -ffp-contract=off keeps every complex product/sum rounded as in the
+# reference (no FMA); packed fp32 ops are disabled for every HIP object.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if out.strip():
+        print(out, flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def engine_path() -> str:
+    return os.path.join(HERE, "libdqsim_gpu.so")
+
+
+def core_path() -> str:
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    return os.path.join(HERE, "_core" + suffix)
+
+
+def build_engine() -> str:
+    out = engine_path()
+    _run([HIPCC, *HIP_FLAGS, os.path.join(HERE, "csrc", "dq_engine.hip"), "-o", out])
+    return out
+
+
+def build_core() -> str:
+    import pybind11
+
+    out = core_path()
+    inc = [f"-I{sysconfig.get_paths()['include']}", f"-I{pybind11.get_include()}"]
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fvisibility=hidden", *inc,
+          os.path.join(HERE, "csrc", "host", "core.cpp"), "-o", out,
+          f"-L{HERE}", "-ldqsim_gpu", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
+def build_all() -> None:
+    build_engine()
+    build_core()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"engine": build_engine, "core": build_core, "all": build_all}[w]()

@@ -1,0 +1,148 @@
+"""ctypes binding of the engine's C ABI (include/dqsim_gpu.h).
+
+The same entry points a Rust/PyO3 host binds through ``extern "C"``; this
+module exposes them to Python for direct use (tests, tools) without the C++
+host.  ``State`` owns a device statevector.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdqsim_gpu.so")
+
+C128, C64 = 0, 1
+ERR_ARG, ERR_OOM, ERR_DEVICE = -1, -2, -3
+
+# every function declared in include/dqsim_gpu.h
+EXPORTS = (
+    "dq_state_create", "dq_state_destroy", "dq_state_num_qubits", "dq_state_set", "dq_state_reset",
+    "dq_apply_1q", "dq_apply_nq", "dq_prob_one", "dq_collapse", "dq_marginal_probs", "dq_inner",
+    "dq_copy_to_host", "dq_synchronize", "dq_set_options", "dq_get_stats", "dq_last_error",
+)
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.dq_state_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.dq_state_destroy.argtypes = [vp]
+        L.dq_state_destroy.restype = None
+        L.dq_state_num_qubits.argtypes = [vp]
+        L.dq_state_set.argtypes = [vp, _dp]
+        L.dq_state_reset.argtypes = [vp]
+        L.dq_apply_1q.argtypes = [vp, _dp, ctypes.c_int, _ip, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int]
+        L.dq_apply_nq.argtypes = [vp, _dp, _ip, ctypes.c_int]
+        L.dq_prob_one.argtypes = [vp, ctypes.c_int, _dp]
+        L.dq_collapse.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.dq_marginal_probs.argtypes = [vp, _ip, ctypes.c_int, _dp]
+        L.dq_inner.argtypes = [vp, _dp, _dp, _dp]
+        L.dq_copy_to_host.argtypes = [vp, _dp]
+        L.dq_synchronize.argtypes = [vp]
+        L.dq_set_options.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.dq_get_stats.argtypes = [vp, u64p, u64p, u64p]
+        L.dq_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+class DqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise DqError(rc, lib().dq_last_error().decode())
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class State:
+    def __init__(self, n: int, dtype: int = C128):
+        self.n = n
+        self.dtype = dtype
+        h = ctypes.c_void_p()
+        _check(lib().dq_state_create(n, dtype, ctypes.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().dq_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, psi: np.ndarray) -> None:
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert psi.size == 1 << self.n
+        _check(lib().dq_state_set(self.h, _d(psi.view(np.float64))))
+
+    def reset(self) -> None:
+        _check(lib().dq_state_reset(self.h))
+
+    def apply_1q(self, u: np.ndarray, target: int, controls=()) -> None:
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        w = (ctypes.c_int * max(1, len(controls)))(*[c[0] for c in controls])
+        v = (ctypes.c_ubyte * max(1, len(controls)))(*[int(c[1]) for c in controls])
+        _check(lib().dq_apply_1q(self.h, _d(u.view(np.float64)), target, w, v, len(controls)))
+
+    def apply_nq(self, u: np.ndarray, qubits) -> None:
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        q = (ctypes.c_int * len(qubits))(*qubits)
+        _check(lib().dq_apply_nq(self.h, _d(u.view(np.float64)), q, len(qubits)))
+
+    def prob_one(self, qubit: int) -> float:
+        p = ctypes.c_double()
+        _check(lib().dq_prob_one(self.h, qubit, ctypes.byref(p)))
+        return p.value
+
+    def collapse(self, qubit: int, outcome: int, norm: float) -> None:
+        _check(lib().dq_collapse(self.h, qubit, outcome, norm))
+
+    def marginal_probs(self, qubits) -> np.ndarray:
+        q = (ctypes.c_int * max(1, len(qubits)))(*qubits)
+        out = np.zeros(1 << len(qubits), dtype=np.float64)
+        _check(lib().dq_marginal_probs(self.h, q, len(qubits), _d(out)))
+        return out
+
+    def inner(self, other: np.ndarray) -> complex:
+        other = np.ascontiguousarray(other, dtype=np.complex128)
+        re, im = ctypes.c_double(), ctypes.c_double()
+        _check(lib().dq_inner(self.h, _d(other.view(np.float64)), ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
+
+    def get(self) -> np.ndarray:
+        out = np.empty(1 << self.n, dtype=np.complex128)
+        _check(lib().dq_copy_to_host(self.h, _d(out.view(np.float64))))
+        return out
+
+    def synchronize(self) -> None:
+        _check(lib().dq_synchronize(self.h))
+
+    def set_options(self, fuse: int = 1, tile_bits: int = 0) -> None:
+        _check(lib().dq_set_options(self.h, fuse, tile_bits))
+
+    def stats(self) -> tuple[int, int, int]:
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().dq_get_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value

@@ -1,0 +1,231 @@
+// dq_device.hpp -- device-side building blocks of the gfx950 statevector engine.
+//
+// Arithmetic contract: every gate update uses the reference's complex op order
+// (num-complex 0.4.6 as called from src/engine.rs:58-59,146-151):
+//   mul (ar*br - ai*bi, ar*bi + ai*br); add componentwise; no contraction.
+// The library is compiled with -ffp-contract=off, so each product and sum is
+// rounded exactly as on the reference.  Specialised op kinds (diagonal,
+// monomial, controlled-2x2) skip only terms whose matrix entry is exactly
+// zero, whose products are +-0 and leave every sum unchanged, so they give
+// the dense formula's values (up to the sign of a zero).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dq {
+
+template <typename T>
+struct cplx {
+    T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+
+enum OpKind : int {
+    K_DENSE1 = 0,   // 2x2 on bit[0]
+    K_DIAG = 1,     // diagonal 2^k, entries at mat
+    K_MONO = 2,     // monomial: out[r] = ph[r] * in[perm[r]], ph at mat, perm at aux
+    K_CTRL1 = 3,    // 2x2 on bit[k-1] where (i & cmask) == cval
+    K_DENSEK = 4,   // dense 2^k x 2^k, acc = 0; acc += U[r][c] * v[c]
+    K_COLLAPSE = 5  // keep bit[0] == outcome (divide by norm), zero the rest
+};
+
+// One queued operation, positions relative to the index space it runs on
+// (global qubits for direct launches, tile-local bits inside a tile pass).
+struct DevOp {
+    int kind;
+    int k;
+    int mat;          // offset (complex entries) into the matrix pool
+    int aux;          // offset (ints) into the perm pool (K_MONO)
+    int bit[5];       // bit[0] = MSB of the gate index
+    int sbit[5];      // the same positions, ascending
+    int outcome;
+    int pad;
+    uint64_t cmask;   // K_CTRL1
+    uint64_t cval;
+    double norm;      // K_COLLAPSE
+};
+
+// Index of the g-th group base: insert a zero at each ascending position.
+template <typename I>
+__device__ __forceinline__ I insert_zeros(I g, const int *sbit, int k) {
+    for (int j = 0; j < k; ++j) {
+        const int p = sbit[j];
+        const I lo = g & ((I(1) << p) - 1);
+        g = ((g >> p) << (p + 1)) | lo;
+    }
+    return g;
+}
+
+template <typename I>
+__device__ __forceinline__ I group_offset(int j, const int *bit, int k) {
+    I o = 0;
+    for (int b = 0; b < k; ++b)
+        if ((j >> (k - 1 - b)) & 1) o |= I(1) << bit[b];
+    return o;
+}
+
+template <typename I>
+__device__ __forceinline__ int gather_bits(I i, const int *bit, int k) {
+    int j = 0;
+    for (int b = 0; b < k; ++b) j = (j << 1) | int((i >> bit[b]) & 1);
+    return j;
+}
+
+// Selects v[idx] with compile-time indexing only, so v stays in registers.
+template <typename T, int DIM>
+__device__ __forceinline__ cplx<T> pick(const cplx<T> (&v)[DIM], int idx) {
+    cplx<T> r = v[0];
+#pragma unroll
+    for (int j = 1; j < DIM; ++j)
+        if (idx == j) r = v[j];
+    return r;
+}
+
+// Monomial gate: out[r] = ph[r] * in[perm[r]] on every group of 2^K.
+template <typename T, typename I, int K>
+__device__ __forceinline__ void mono_k(cplx<T> *__restrict__ a, int nbits, const DevOp &op,
+                                       const cplx<T> *__restrict__ mats,
+                                       const int *__restrict__ perms, I tid, I nthr) {
+    constexpr int DIM = 1 << K;
+    I off[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) off[j] = group_offset<I>(j, op.bit, K);
+    int pm[DIM];
+    cplx<T> ph[DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+        pm[r] = perms[op.aux + r];
+        ph[r] = mats[op.mat + r];
+    }
+    const I ngroups = I(1) << (nbits - K);
+    for (I g = tid; g < ngroups; g += nthr) {
+        const I base = insert_zeros<I>(g, op.sbit, K);
+        cplx<T> v[DIM];
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) v[j] = a[base | off[j]];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) a[base | off[r]] = cmul(ph[r], pick<T, DIM>(v, pm[r]));
+    }
+}
+
+// Dense gate, the reference's row loop: acc = 0; acc += U[r][c] * v[c].
+template <typename T, typename I, int K>
+__device__ __forceinline__ void dense_k(cplx<T> *__restrict__ a, int nbits, const DevOp &op,
+                                        const cplx<T> *__restrict__ mats, I tid, I nthr) {
+    constexpr int DIM = 1 << K;
+    I off[DIM];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) off[j] = group_offset<I>(j, op.bit, K);
+    const cplx<T> *__restrict__ U = mats + op.mat;
+    const I ngroups = I(1) << (nbits - K);
+    for (I g = tid; g < ngroups; g += nthr) {
+        const I base = insert_zeros<I>(g, op.sbit, K);
+        cplx<T> v[DIM];
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) v[j] = a[base | off[j]];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+            cplx<T> acc{T(0), T(0)};
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) acc = cadd(acc, cmul(U[r * DIM + c], v[c]));
+            a[base | off[r]] = acc;
+        }
+    }
+}
+
+// Applies one op to the 2^nbits amplitudes at `a` (LDS tile or global
+// state), work items distributed over `nthr` threads starting at `tid`.
+template <typename T, typename I>
+__device__ __forceinline__ void apply_op(cplx<T> *__restrict__ a, int nbits, const DevOp &op,
+                                         const cplx<T> *__restrict__ mats,
+                                         const int *__restrict__ perms, I tid, I nthr) {
+    const int k = op.k;
+    switch (op.kind) {
+    case K_DENSE1: {
+        const int p = op.bit[0];
+        const I half = I(1) << p;
+        const I npairs = I(1) << (nbits - 1);
+        const cplx<T> u00 = mats[op.mat], u01 = mats[op.mat + 1];
+        const cplx<T> u10 = mats[op.mat + 2], u11 = mats[op.mat + 3];
+        for (I g = tid; g < npairs; g += nthr) {
+            const I i0 = ((g >> p) << (p + 1)) | (g & (half - 1));
+            const I i1 = i0 | half;
+            const cplx<T> x = a[i0], y = a[i1];
+            a[i0] = cadd(cmul(u00, x), cmul(u01, y));
+            a[i1] = cadd(cmul(u10, x), cmul(u11, y));
+        }
+        break;
+    }
+    case K_CTRL1: {
+        const int p = op.bit[k - 1];
+        const I half = I(1) << p;
+        const I npairs = I(1) << (nbits - 1);
+        const I cm = I(op.cmask), cv = I(op.cval);
+        const cplx<T> u00 = mats[op.mat], u01 = mats[op.mat + 1];
+        const cplx<T> u10 = mats[op.mat + 2], u11 = mats[op.mat + 3];
+        for (I g = tid; g < npairs; g += nthr) {
+            const I i0 = ((g >> p) << (p + 1)) | (g & (half - 1));
+            if ((i0 & cm) != cv) continue;
+            const I i1 = i0 | half;
+            const cplx<T> x = a[i0], y = a[i1];
+            a[i0] = cadd(cmul(u00, x), cmul(u01, y));
+            a[i1] = cadd(cmul(u10, x), cmul(u11, y));
+        }
+        break;
+    }
+    case K_DIAG: {
+        const I total = I(1) << nbits;
+        for (I i = tid; i < total; i += nthr) {
+            const int j = gather_bits<I>(i, op.bit, k);
+            a[i] = cmul(mats[op.mat + j], a[i]);
+        }
+        break;
+    }
+    case K_COLLAPSE: {
+        const I total = I(1) << nbits;
+        const int p = op.bit[0];
+        const T nrm = T(op.norm);
+        for (I i = tid; i < total; i += nthr) {
+            if (int((i >> p) & 1) == op.outcome) {
+                cplx<T> v = a[i];
+                v.re = v.re / nrm;
+                v.im = v.im / nrm;
+                a[i] = v;
+            } else {
+                a[i] = cplx<T>{T(0), T(0)};
+            }
+        }
+        break;
+    }
+    case K_MONO:
+        switch (k) {
+        case 1: mono_k<T, I, 1>(a, nbits, op, mats, perms, tid, nthr); break;
+        case 2: mono_k<T, I, 2>(a, nbits, op, mats, perms, tid, nthr); break;
+        case 3: mono_k<T, I, 3>(a, nbits, op, mats, perms, tid, nthr); break;
+        case 4: mono_k<T, I, 4>(a, nbits, op, mats, perms, tid, nthr); break;
+        default: mono_k<T, I, 5>(a, nbits, op, mats, perms, tid, nthr); break;
+        }
+        break;
+    case K_DENSEK:
+        switch (k) {
+        case 1: dense_k<T, I, 1>(a, nbits, op, mats, tid, nthr); break;
+        case 2: dense_k<T, I, 2>(a, nbits, op, mats, tid, nthr); break;
+        case 3: dense_k<T, I, 3>(a, nbits, op, mats, tid, nthr); break;
+        case 4: dense_k<T, I, 4>(a, nbits, op, mats, tid, nthr); break;
+        default: dense_k<T, I, 5>(a, nbits, op, mats, tid, nthr); break;
+        }
+        break;
+    default:
+        break;
+    }
+}
+
+}  // namespace dq

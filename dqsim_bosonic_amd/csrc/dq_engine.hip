@@ -1,0 +1,1060 @@
+// dq_engine.hip -- gfx950 statevector engine behind the C ABI of
+// include/dqsim_gpu.h.  Replaces the Rayon loops of the reference's
+// src/engine.rs (apply_one_qubit :24-101, apply_n_qubit :109-181,
+// measure_qubit :193-257, marginal_probs :368-390) and the fidelity fold of
+// src/monolithic/statevector/simulator.rs:184-191.
+//
+// Design:
+//  * Gate calls are classified on the host (diagonal / monomial /
+//    controlled-2x2 / dense) and queued on the handle.
+//  * At a flush the queue is cut into passes.  A pass picks a set S of
+//    `tile_bits` qubits (the low `low_bits` qubits always included, so each
+//    tile is made of contiguous 2^low_bits-amplitude runs and loads coalesce)
+//    and takes, in program order, every queued op whose qubits fit in S and do
+//    not touch a qubit held back by an earlier op left for a later pass
+//    (fusion level 2, the default).  Ops on disjoint qubits commute, so the
+//    result equals program order in exact arithmetic; the rounding of the
+//    reordered products may differ in the last bit.  Fusion level 1 keeps
+//    strict program order (each pass is a prefix of the queue) and so keeps
+//    the reference's rounding bit for bit; level 0 issues one sweep per op.
+//  * One pass = one kernel = one HBM sweep: each workgroup loads a 2^tile_bits
+//    amplitude tile into LDS, applies all of the pass's ops in order with the
+//    reference's arithmetic, and writes the tile back.
+//  * Reductions (p1, marginals, inner products) are two-level: per-block tree
+//    sums (wavefront shuffles + LDS) into a fixed number of partials, then one
+//    block sums the partials in a fixed order; results are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dqsim_gpu.h"
+#include "dq_device.hpp"
+
+using namespace dq;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return fail(DQ_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+constexpr int kThreads = 256;       // workgroup size of every kernel
+constexpr int kReduceBlocks = 1024;  // partial sums per reduction (fixed => deterministic)
+
+// ------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fill_basis(cplx<T> *amps, uint64_t count) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+        amps[i] = cplx<T>{T(i == 0 ? 1 : 0), T(0)};
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_from_f64(cplx<T> *dst, const cplx<double> *src,
+                                                       uint64_t count) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+        dst[i] = cplx<T>{T(src[i].re), T(src[i].im)};
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_to_f64(cplx<double> *dst, const cplx<T> *src,
+                                                     uint64_t count) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+        dst[i] = cplx<double>{double(src[i].re), double(src[i].im)};
+}
+
+// One op over the whole state in global memory (unfused mode).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_direct_op(cplx<T> *amps, int n, const DevOp *ops,
+                                                        int op_index, const cplx<T> *mats,
+                                                        const int *perms) {
+    const DevOp op = ops[op_index];
+    const uint64_t tid = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t nthr = uint64_t(gridDim.x) * blockDim.x;
+    apply_op<T, uint64_t>(amps, n, op, mats, perms, tid, nthr);
+}
+
+// Tile pass descriptor, passed by value (kernarg).
+struct PassDesc {
+    int b;         // tile bits
+    int n_outer;   // n - b
+    int op_begin;  // first op of the pass
+    int op_count;
+    int tq[16];    // tile bit -> qubit (tq[j] = j for j < low_bits)
+    int oq[48];    // outer bit -> qubit
+};
+
+// One HBM sweep: load tile -> apply ops in LDS -> store tile.
+// Tile-local index i = s * 256 + tid; low 8 bits from the thread, the rest
+// from the slot s, so each thread's global offsets split into a per-thread
+// part and a per-slot part computed once per launch.
+template <typename T, int NSLOT>
+__global__ __launch_bounds__(kThreads) void k_tile_pass(cplx<T> *__restrict__ amps, PassDesc pd,
+                                                        const DevOp *__restrict__ ops,
+                                                        const cplx<T> *__restrict__ mats,
+                                                        const int *__restrict__ perms,
+                                                        uint64_t ntiles) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cplx<T> *sh = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int tid = threadIdx.x;
+    const int b = pd.b;
+    const int lowb = b < 8 ? b : 8;
+    const bool active = tid < (1 << lowb);
+    uint64_t off_lo = 0;
+    for (int j = 0; j < lowb; ++j)
+        if ((tid >> j) & 1) off_lo |= uint64_t(1) << pd.tq[j];
+    uint64_t off_hi[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+        uint64_t o = 0;
+        for (int j = 8; j < b; ++j)
+            if ((s >> (j - 8)) & 1) o |= uint64_t(1) << pd.tq[j];
+        off_hi[s] = o;
+    }
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint64_t base = 0;
+        for (int j = 0; j < pd.n_outer; ++j)
+            if ((t >> j) & 1) base |= uint64_t(1) << pd.oq[j];
+        if (active) {
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s) sh[s * kThreads + tid] = amps[base + off_lo + off_hi[s]];
+        }
+        __syncthreads();
+        for (int o = 0; o < pd.op_count; ++o) {
+            const DevOp op = ops[pd.op_begin + o];
+            apply_op<T, uint32_t>(sh, b, op, mats, perms, uint32_t(tid), uint32_t(kThreads));
+            __syncthreads();
+        }
+        if (active) {
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s) amps[base + off_lo + off_hi[s]] = sh[s * kThreads + tid];
+        }
+        // The next tile's loads write only this thread's own slots, which it
+        // has just read, so no barrier is needed here.
+    }
+}
+
+// ---- reductions ----
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Fixed-shape block reduction: wave butterflies, then wave 0 sums the four
+// wave totals in order.  Result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *scratch) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) scratch[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int j = 0; j < kThreads / 64; ++j) r += scratch[j];
+    __syncthreads();
+    return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_prob_one(const cplx<T> *__restrict__ amps, int n, int q,
+                                                       double *__restrict__ partials) {
+    __shared__ double scratch[kThreads / 64];
+    const uint64_t half = uint64_t(1) << q;
+    const uint64_t count = uint64_t(1) << (n - 1);
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    double s = 0.0;
+    for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < count; g += stride) {
+        const uint64_t i = ((g >> q) << (q + 1)) | half | (g & (half - 1));
+        const cplx<T> a = amps[i];
+        const double re = double(a.re), im = double(a.im);
+        s += re * re + im * im;
+    }
+    const double r = block_sum(s, scratch);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// Sums `count` partials (one block, fixed order) -> out[slot].
+__global__ __launch_bounds__(kThreads) void k_sum_partials(const double *__restrict__ partials, int count,
+                                                           int stride_elems, double *__restrict__ out) {
+    __shared__ double scratch[kThreads / 64];
+    for (int c = 0; c < stride_elems; ++c) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < count; i += blockDim.x) s += partials[size_t(i) * stride_elems + c];
+        const double r = block_sum(s, scratch);
+        if (threadIdx.x == 0) out[c] = r;
+    }
+}
+
+// out[bin] = sum over chunks of partials[chunk * bins + bin], ascending chunks.
+__global__ __launch_bounds__(kThreads) void k_sum_chunks(const double *__restrict__ partials, int chunks,
+                                                         uint64_t bins, double *__restrict__ out) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < bins; j += stride) {
+        double s = 0.0;
+        for (int c = 0; c < chunks; ++c) s += partials[uint64_t(c) * bins + j];
+        out[j] = s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_inner(const cplx<T> *__restrict__ amps,
+                                                    const cplx<double> *__restrict__ other, uint64_t count,
+                                                    double *__restrict__ partials) {
+    __shared__ double scratch[kThreads / 64];
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    double sr = 0.0, si = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const double ar = double(amps[i].re), ai = -double(amps[i].im);
+        const cplx<double> b = other[i];
+        sr += ar * b.re - ai * b.im;
+        si += ar * b.im + ai * b.re;
+    }
+    const double r0 = block_sum(sr, scratch);
+    const double r1 = block_sum(si, scratch);
+    if (threadIdx.x == 0) {
+        partials[2 * size_t(blockIdx.x)] = r0;
+        partials[2 * size_t(blockIdx.x) + 1] = r1;
+    }
+}
+
+// Full distribution over all n qubits in a given order: out[j(i)] = |psi_i|^2.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_probs_perm(const cplx<T> *__restrict__ amps, int n,
+                                                         const int *__restrict__ qubits,
+                                                         double *__restrict__ out, int identity) {
+    const uint64_t count = uint64_t(1) << n;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const double re = double(amps[i].re), im = double(amps[i].im);
+        uint64_t j = i;
+        if (!identity) {
+            j = 0;
+            for (int b = 0; b < n; ++b) j = (j << 1) | ((i >> qubits[b]) & 1);
+        }
+        out[j] = re * re + im * im;
+    }
+}
+
+// Marginal with few terms per bin: one thread per bin sums its terms in
+// ascending index order (the reference's order, src/engine.rs:384-387).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_marginal_seq(const cplx<T> *__restrict__ amps, int n,
+                                                           const int *__restrict__ qubits, int k,
+                                                           const int *__restrict__ rest, double *__restrict__ out) {
+    const uint64_t bins = uint64_t(1) << k;
+    const uint64_t terms = uint64_t(1) << (n - k);
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < bins; j += stride) {
+        uint64_t base = 0;
+        for (int b = 0; b < k; ++b)
+            if ((j >> (k - 1 - b)) & 1) base |= uint64_t(1) << qubits[b];
+        double s = 0.0;
+        for (uint64_t r = 0; r < terms; ++r) {
+            uint64_t i = base;
+            for (int b = 0; b < n - k; ++b)
+                if ((r >> b) & 1) i |= uint64_t(1) << rest[b];
+            const double re = double(amps[i].re), im = double(amps[i].im);
+            s += re * re + im * im;
+        }
+        out[j] = s;
+    }
+}
+
+// Marginal with many terms per bin: block (bin, chunk) tree-sums a chunk of
+// the bin's terms into partials[chunk * bins + bin].
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_marginal_tree(const cplx<T> *__restrict__ amps, int n,
+                                                            const int *__restrict__ qubits, int k,
+                                                            const int *__restrict__ rest, int chunks,
+                                                            double *__restrict__ partials) {
+    __shared__ double scratch[kThreads / 64];
+    const uint64_t bins = uint64_t(1) << k;
+    const uint64_t terms = uint64_t(1) << (n - k);
+    const uint64_t bin = blockIdx.x % bins;
+    const uint64_t chunk = blockIdx.x / bins;
+    uint64_t base = 0;
+    for (int b = 0; b < k; ++b)
+        if ((bin >> (k - 1 - b)) & 1) base |= uint64_t(1) << qubits[b];
+    const uint64_t per = terms / chunks;
+    double s = 0.0;
+    for (uint64_t r = chunk * per + threadIdx.x; r < (chunk + 1) * per; r += blockDim.x) {
+        uint64_t i = base;
+        for (int b = 0; b < n - k; ++b)
+            if ((r >> b) & 1) i |= uint64_t(1) << rest[b];
+        const double re = double(amps[i].re), im = double(amps[i].im);
+        s += re * re + im * im;
+    }
+    const double v = block_sum(s, scratch);
+    if (threadIdx.x == 0) partials[chunk * bins + bin] = v;
+}
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+
+struct HostOp {
+    int kind = K_DENSE1;
+    int k = 1;
+    int q[5] = {0, 0, 0, 0, 0};  // global qubits, q[0] = MSB of the gate index
+    uint64_t cmask = 0, cval = 0;  // global masks (K_CTRL1)
+    int outcome = 0;
+    double norm = 1.0;
+    std::vector<double> mat;  // (re, im) pairs: 4 (2x2), 2^k (diag/mono), 4^k (dense)
+    std::vector<int> perm;    // K_MONO
+    uint64_t qmask() const {
+        uint64_t m = 0;
+        for (int j = 0; j < k; ++j) m |= uint64_t(1) << q[j];
+        return m;
+    }
+};
+
+struct Pass {
+    uint64_t S = 0;  // tile qubits (mask); 0 = direct launch of a single op
+    std::vector<int> ops;
+};
+
+int popcount64(uint64_t x) { return __builtin_popcountll(x); }
+
+}  // namespace
+
+struct dq_state {
+    int n = 0;
+    int dtype = DQ_C128;
+    int device = 0;
+    size_t elem = 16;
+    void *amps = nullptr;
+    hipStream_t stream = nullptr;
+    int fuse = 2;
+    int tile_bits = 12;
+    int low_bits = 5;
+    std::vector<HostOp> queue;
+    // upload buffers
+    void *d_upload = nullptr;
+    size_t d_upload_cap = 0;
+    unsigned char *h_upload = nullptr;
+    size_t h_upload_cap = 0;
+    hipEvent_t upload_done = nullptr;
+    // reduction buffers
+    double *d_partials = nullptr;
+    double *h_result = nullptr;  // pinned, 64 doubles
+    double *d_result = nullptr;
+    // stats
+    uint64_t sweeps = 0, launches = 0, gates = 0;
+};
+
+namespace {
+
+uint64_t amp_count(const dq_state *s) { return uint64_t(1) << s->n; }
+
+int grid_for(uint64_t work) {
+    uint64_t g = (work + kThreads - 1) / kThreads;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return int(g);
+}
+
+int check_qubit(const dq_state *s, int q) {
+    if (q < 0 || q >= s->n) return fail(DQ_ERR_ARG, "qubit index " + std::to_string(q) +
+                                                        " out of range for " + std::to_string(s->n) +
+                                                        "-qubit state");
+    return DQ_OK;
+}
+
+bool is_zero(const double *c) { return c[0] == 0.0 && c[1] == 0.0; }
+
+// Classifies a dense 2^k matrix into the cheapest op kind that reproduces the
+// dense formula exactly (see dq_device.hpp): diagonal (every off-diagonal
+// entry is exactly zero), monomial (one nonzero per row, a permutation),
+// controlled-2x2 (identity outside the bottom-right 2x2 block), else dense.
+void classify(HostOp &op, const double *u) {
+    const int dim = 1 << op.k;
+    auto at = [&](int r, int c) { return u + 2 * (size_t(r) * dim + c); };
+    bool diag = true, mono = true;
+    std::vector<int> perm(dim, -1), colcount(dim, 0);
+    for (int r = 0; r < dim; ++r) {
+        int nz = 0;
+        for (int c = 0; c < dim; ++c) {
+            if (is_zero(at(r, c))) continue;
+            ++nz;
+            perm[r] = c;
+            if (c != r) diag = false;
+        }
+        if (nz != 1) mono = false;
+        else colcount[perm[r]]++;
+    }
+    for (int c = 0; c < dim && mono; ++c)
+        if (colcount[c] != 1) mono = false;
+    if (diag) {
+        op.kind = K_DIAG;
+        op.mat.resize(2 * dim);
+        for (int r = 0; r < dim; ++r) {
+            op.mat[2 * r] = at(r, r)[0];
+            op.mat[2 * r + 1] = at(r, r)[1];
+        }
+        return;
+    }
+    if (mono) {
+        op.kind = K_MONO;
+        op.mat.resize(2 * dim);
+        op.perm = perm;
+        for (int r = 0; r < dim; ++r) {
+            op.mat[2 * r] = at(r, perm[r])[0];
+            op.mat[2 * r + 1] = at(r, perm[r])[1];
+        }
+        return;
+    }
+    if (op.k == 1) {
+        op.kind = K_DENSE1;
+        op.mat.assign(u, u + 8);
+        return;
+    }
+    // controlled 2x2: identity except the bottom-right 2x2 block
+    bool ctrl = true;
+    for (int r = 0; r < dim && ctrl; ++r)
+        for (int c = 0; c < dim; ++c) {
+            const bool in_block = r >= dim - 2 && c >= dim - 2;
+            if (in_block) continue;
+            const double *e = at(r, c);
+            const bool want_one = (r == c);
+            if (want_one ? !(e[0] == 1.0 && e[1] == 0.0) : !is_zero(e)) { ctrl = false; break; }
+        }
+    if (ctrl) {
+        op.kind = K_CTRL1;
+        op.mat.resize(8);
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < 2; ++c) {
+                op.mat[2 * (2 * r + c)] = at(dim - 2 + r, dim - 2 + c)[0];
+                op.mat[2 * (2 * r + c) + 1] = at(dim - 2 + r, dim - 2 + c)[1];
+            }
+        op.cmask = op.cval = 0;
+        for (int j = 0; j < op.k - 1; ++j) {
+            op.cmask |= uint64_t(1) << op.q[j];
+            op.cval |= uint64_t(1) << op.q[j];
+        }
+        return;
+    }
+    op.kind = K_DENSEK;
+    op.mat.assign(u, u + 2 * size_t(dim) * dim);
+}
+
+// ---- upload + launch ----
+
+struct Packed {
+    std::vector<DevOp> ops;
+    std::vector<double> mats;  // complex entries as (re, im)
+    std::vector<int> perms;
+};
+
+// Converts a host op to a device op with positions through `pos` (qubit ->
+// position in the index space of the launch).
+DevOp to_dev(const HostOp &h, const int *pos, Packed &pk) {
+    DevOp d;
+    std::memset(&d, 0, sizeof d);
+    d.kind = h.kind;
+    d.k = h.k;
+    d.mat = int(pk.mats.size() / 2);
+    pk.mats.insert(pk.mats.end(), h.mat.begin(), h.mat.end());
+    d.aux = int(pk.perms.size());
+    pk.perms.insert(pk.perms.end(), h.perm.begin(), h.perm.end());
+    for (int j = 0; j < h.k; ++j) d.bit[j] = pos[h.q[j]];
+    for (int j = 0; j < h.k; ++j) d.sbit[j] = d.bit[j];
+    std::sort(d.sbit, d.sbit + h.k);
+    d.outcome = h.outcome;
+    d.norm = h.norm;
+    d.cmask = d.cval = 0;
+    for (int j = 0; j < h.k - 1 && h.kind == K_CTRL1; ++j) {
+        const uint64_t gbit = uint64_t(1) << h.q[j];
+        d.cmask |= uint64_t(1) << pos[h.q[j]];
+        if (h.cval & gbit) d.cval |= uint64_t(1) << pos[h.q[j]];
+    }
+    return d;
+}
+
+int ensure_device_buffer(dq_state *s, size_t bytes) {
+    if (bytes <= s->d_upload_cap) return DQ_OK;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->d_upload) HIP_TRY(hipFree(s->d_upload));
+    s->d_upload = nullptr;
+    size_t cap = std::max<size_t>(bytes, 1 << 20);
+    if (hipMalloc(&s->d_upload, cap) != hipSuccess) {
+        s->d_upload_cap = 0;
+        return fail(DQ_ERR_OOM, "hipMalloc of op buffer failed");
+    }
+    s->d_upload_cap = cap;
+    return DQ_OK;
+}
+
+int ensure_host_buffer(dq_state *s, size_t bytes) {
+    if (bytes <= s->h_upload_cap) return DQ_OK;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->h_upload) HIP_TRY(hipHostFree(s->h_upload));
+    s->h_upload = nullptr;
+    size_t cap = std::max<size_t>(bytes, 1 << 20);
+    if (hipHostMalloc(reinterpret_cast<void **>(&s->h_upload), cap, hipHostMallocDefault) != hipSuccess) {
+        s->h_upload_cap = 0;
+        return fail(DQ_ERR_OOM, "hipHostMalloc of staging buffer failed");
+    }
+    s->h_upload_cap = cap;
+    return DQ_OK;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Copies ops/mats/perms to the device; returns device pointers.
+int upload(dq_state *s, const Packed &pk, DevOp **d_ops, void **d_mats, int **d_perms) {
+    const size_t ops_b = align_up(pk.ops.size() * sizeof(DevOp), 256);
+    const size_t real_b = (s->dtype == DQ_C128) ? 8 : 4;
+    const size_t mats_b = align_up(pk.mats.size() * real_b, 256);
+    const size_t perms_b = align_up(pk.perms.size() * sizeof(int), 256);
+    const size_t total = ops_b + mats_b + perms_b + 256;
+    // the previous upload must have been consumed before the staging area is reused
+    HIP_TRY(hipEventSynchronize(s->upload_done));
+    int rc = ensure_host_buffer(s, total);
+    if (rc) return rc;
+    rc = ensure_device_buffer(s, total);
+    if (rc) return rc;
+    unsigned char *h = s->h_upload;
+    std::memcpy(h, pk.ops.data(), pk.ops.size() * sizeof(DevOp));
+    if (s->dtype == DQ_C128) {
+        std::memcpy(h + ops_b, pk.mats.data(), pk.mats.size() * 8);
+    } else {
+        float *f = reinterpret_cast<float *>(h + ops_b);
+        for (size_t i = 0; i < pk.mats.size(); ++i) f[i] = float(pk.mats[i]);
+    }
+    std::memcpy(h + ops_b + mats_b, pk.perms.data(), pk.perms.size() * sizeof(int));
+    HIP_TRY(hipMemcpyAsync(s->d_upload, h, total, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipEventRecord(s->upload_done, s->stream));
+    unsigned char *d = static_cast<unsigned char *>(s->d_upload);
+    *d_ops = reinterpret_cast<DevOp *>(d);
+    *d_mats = d + ops_b;
+    *d_perms = reinterpret_cast<int *>(d + ops_b + mats_b);
+    return DQ_OK;
+}
+
+template <typename T>
+int launch_tile(dq_state *s, const PassDesc &pd, const DevOp *ops, const void *mats, const int *perms) {
+    const uint64_t ntiles = uint64_t(1) << pd.n_outer;
+    const int nslot = pd.b > 8 ? 1 << (pd.b - 8) : 1;
+    const size_t shmem = sizeof(cplx<T>) * size_t(nslot) * kThreads;
+    // persistent-style grid: enough workgroups for every CU several times over
+    const uint64_t grid = std::min<uint64_t>(ntiles, 8192);
+    const cplx<T> *m = static_cast<const cplx<T> *>(mats);
+    cplx<T> *a = static_cast<cplx<T> *>(s->amps);
+#define DQ_TILE_CASE(NS)                                                                       \
+    case NS:                                                                                   \
+        if (shmem > 65536)                                                                     \
+            HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_pass<T, NS>),   \
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem))); \
+        hipLaunchKernelGGL((k_tile_pass<T, NS>), dim3(grid), dim3(kThreads), shmem, s->stream, a, pd, \
+                           ops, m, perms, ntiles);                                             \
+        break;
+    switch (nslot) {
+        DQ_TILE_CASE(1)
+        DQ_TILE_CASE(2)
+        DQ_TILE_CASE(4)
+        DQ_TILE_CASE(8)
+        DQ_TILE_CASE(16)
+        DQ_TILE_CASE(32)
+    default:
+        return fail(DQ_ERR_ARG, "unsupported tile size");
+    }
+#undef DQ_TILE_CASE
+    HIP_TRY(hipGetLastError());
+    s->launches++;
+    s->sweeps++;
+    return DQ_OK;
+}
+
+template <typename T>
+int launch_direct(dq_state *s, const DevOp *ops, int index, const void *mats, const int *perms) {
+    const uint64_t work = amp_count(s) / 2;
+    hipLaunchKernelGGL(k_direct_op<T>, dim3(grid_for(work)), dim3(kThreads), 0, s->stream,
+                       static_cast<cplx<T> *>(s->amps), s->n, ops, index,
+                       static_cast<const cplx<T> *>(mats), perms);
+    HIP_TRY(hipGetLastError());
+    s->launches++;
+    s->sweeps++;
+    return DQ_OK;
+}
+
+// Greedy pass builder (see the file header).
+std::vector<Pass> schedule(const dq_state *s) {
+    std::vector<Pass> passes;
+    const int n = s->n;
+    const int b = std::min(s->tile_bits, n);
+    // leave room for a 5-qubit gate on qubits outside the low block
+    const int low = b < n ? std::min(s->low_bits, b - 5) : b;
+    const uint64_t low_mask = (low >= 64) ? ~uint64_t(0) : ((uint64_t(1) << low) - 1);
+    std::vector<int> remaining(s->queue.size());
+    for (size_t i = 0; i < remaining.size(); ++i) remaining[i] = int(i);
+    while (!remaining.empty()) {
+        Pass p;
+        p.S = low_mask;
+        uint64_t blocked = 0;
+        std::vector<int> rest;
+        for (int idx : remaining) {
+            const uint64_t qm = s->queue[idx].qmask();
+            if (qm & blocked) {
+                blocked |= qm;
+                rest.push_back(idx);
+                continue;
+            }
+            if (popcount64(p.S | qm) <= b) {
+                p.S |= qm;
+                p.ops.push_back(idx);
+            } else {
+                blocked |= qm;
+                rest.push_back(idx);
+                // program-order mode: the pass is a prefix of the queue
+                if (s->fuse == 1) blocked = ~uint64_t(0);
+            }
+        }
+        // fill the tile up to b qubits with the lowest unused qubits
+        for (int q = 0; q < n && popcount64(p.S) < b; ++q) p.S |= uint64_t(1) << q;
+        passes.push_back(std::move(p));
+        remaining.swap(rest);
+    }
+    return passes;
+}
+
+template <typename T>
+int flush_t(dq_state *s) {
+    if (s->queue.empty()) return DQ_OK;
+    const int n = s->n;
+    Packed pk;
+    std::vector<PassDesc> descs;
+    int rc = DQ_OK;
+    if (s->fuse && n >= 1) {
+        std::vector<Pass> passes = schedule(s);
+        for (const Pass &p : passes) {
+            PassDesc pd;
+            std::memset(&pd, 0, sizeof pd);
+            int pos[64];
+            int tb = 0, ob = 0;
+            for (int q = 0; q < n; ++q) {
+                if ((p.S >> q) & 1) {
+                    pd.tq[tb] = q;
+                    pos[q] = tb++;
+                } else {
+                    pd.oq[ob++] = q;
+                    pos[q] = -1;
+                }
+            }
+            pd.b = tb;
+            pd.n_outer = ob;
+            pd.op_begin = int(pk.ops.size());
+            pd.op_count = int(p.ops.size());
+            for (int idx : p.ops) pk.ops.push_back(to_dev(s->queue[idx], pos, pk));
+            descs.push_back(pd);
+        }
+    } else {
+        int pos[64];
+        for (int q = 0; q < 64; ++q) pos[q] = q;
+        for (const HostOp &h : s->queue) pk.ops.push_back(to_dev(h, pos, pk));
+    }
+    DevOp *d_ops;
+    void *d_mats;
+    int *d_perms;
+    rc = upload(s, pk, &d_ops, &d_mats, &d_perms);
+    if (rc) return rc;
+    if (s->fuse) {
+        for (const PassDesc &pd : descs) {
+            rc = launch_tile<T>(s, pd, d_ops, d_mats, d_perms);
+            if (rc) return rc;
+        }
+    } else {
+        for (size_t i = 0; i < pk.ops.size(); ++i) {
+            rc = launch_direct<T>(s, d_ops, int(i), d_mats, d_perms);
+            if (rc) return rc;
+        }
+    }
+    s->queue.clear();
+    return DQ_OK;
+}
+
+int flush(dq_state *s) {
+    return s->dtype == DQ_C128 ? flush_t<double>(s) : flush_t<float>(s);
+}
+
+int enqueue(dq_state *s, HostOp &&op) {
+    s->queue.push_back(std::move(op));
+    s->gates++;
+    if (s->queue.size() >= 4096) return flush(s);
+    return DQ_OK;
+}
+
+int fetch_result(dq_state *s, int count) {
+    HIP_TRY(hipMemcpyAsync(s->h_result, s->d_result, sizeof(double) * count, hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return DQ_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------
+
+extern "C" {
+
+const char *dq_last_error(void) { return g_err.c_str(); }
+
+int dq_state_create(int n_qubits, int dtype, dq_state **out) {
+    if (!out) return fail(DQ_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (n_qubits < 0 || n_qubits > 40) return fail(DQ_ERR_ARG, "n_qubits must be in [0, 40]");
+    if (dtype != DQ_C128 && dtype != DQ_C64) return fail(DQ_ERR_ARG, "unknown dtype");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(DQ_ERR_DEVICE, "no HIP device available");
+    dq_state *s = new dq_state();
+    s->n = n_qubits;
+    s->dtype = dtype;
+    s->elem = dtype == DQ_C128 ? 16 : 8;
+    s->tile_bits = dtype == DQ_C128 ? 12 : 13;
+    HIP_TRY(hipGetDevice(&s->device));
+    auto cleanup = [&](int code, const std::string &msg) {
+        dq_state_destroy(s);
+        return fail(code, msg);
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(DQ_ERR_DEVICE, "hipStreamCreate failed");
+    if (hipEventCreateWithFlags(&s->upload_done, hipEventDisableTiming) != hipSuccess)
+        return cleanup(DQ_ERR_DEVICE, "hipEventCreate failed");
+    if (hipMalloc(&s->amps, s->elem * amp_count(s)) != hipSuccess) {
+        s->amps = nullptr;
+        return cleanup(DQ_ERR_OOM, "hipMalloc of " + std::to_string(s->elem * amp_count(s)) +
+                                       " bytes for the statevector failed");
+    }
+    if (hipMalloc(reinterpret_cast<void **>(&s->d_partials), sizeof(double) * 2 * kReduceBlocks * 64) !=
+            hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&s->d_result), sizeof(double) * 64) != hipSuccess)
+        return cleanup(DQ_ERR_OOM, "hipMalloc of reduction buffers failed");
+    if (hipHostMalloc(reinterpret_cast<void **>(&s->h_result), sizeof(double) * 64, hipHostMallocDefault) !=
+        hipSuccess)
+        return cleanup(DQ_ERR_OOM, "hipHostMalloc failed");
+    // KFD allocations are write-combined staging; make sure the event exists
+    // in a signalled state before the first upload waits on it.
+    if (hipEventRecord(s->upload_done, s->stream) != hipSuccess)
+        return cleanup(DQ_ERR_DEVICE, "hipEventRecord failed");
+    int rc = dq_state_reset(s);
+    if (rc) {
+        std::string m = g_err;
+        return cleanup(rc, m);
+    }
+    *out = s;
+    return DQ_OK;
+}
+
+void dq_state_destroy(dq_state *s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->amps) (void)hipFree(s->amps);
+    if (s->d_upload) (void)hipFree(s->d_upload);
+    if (s->h_upload) (void)hipHostFree(s->h_upload);
+    if (s->d_partials) (void)hipFree(s->d_partials);
+    if (s->d_result) (void)hipFree(s->d_result);
+    if (s->h_result) (void)hipHostFree(s->h_result);
+    if (s->upload_done) (void)hipEventDestroy(s->upload_done);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int dq_state_num_qubits(const dq_state *s) { return s ? s->n : -1; }
+
+int dq_state_reset(dq_state *s) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    s->queue.clear();
+    const uint64_t count = amp_count(s);
+    if (s->dtype == DQ_C128)
+        hipLaunchKernelGGL(k_fill_basis<double>, dim3(grid_for(count)), dim3(kThreads), 0, s->stream,
+                           static_cast<cplx<double> *>(s->amps), count);
+    else
+        hipLaunchKernelGGL(k_fill_basis<float>, dim3(grid_for(count)), dim3(kThreads), 0, s->stream,
+                           static_cast<cplx<float> *>(s->amps), count);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return DQ_OK;
+}
+
+int dq_state_set(dq_state *s, const double *src) {
+    if (!s || !src) return fail(DQ_ERR_ARG, "null argument");
+    s->queue.clear();
+    const uint64_t count = amp_count(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->dtype == DQ_C128) {
+        HIP_TRY(hipMemcpy(s->amps, src, 16 * count, hipMemcpyHostToDevice));
+        return DQ_OK;
+    }
+    void *tmp = nullptr;
+    if (hipMalloc(&tmp, 16 * count) != hipSuccess) return fail(DQ_ERR_OOM, "hipMalloc failed");
+    hipError_t e = hipMemcpy(tmp, src, 16 * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_from_f64<float>, dim3(grid_for(count)), dim3(kThreads), 0, s->stream,
+                           static_cast<cplx<float> *>(s->amps), static_cast<const cplx<double> *>(tmp), count);
+        e = hipStreamSynchronize(s->stream);
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(DQ_ERR_DEVICE, hipGetErrorString(e));
+    return DQ_OK;
+}
+
+int dq_apply_1q(dq_state *s, const double u[8], int target, const int *ctrl_wires,
+                const unsigned char *ctrl_vals, int n_ctrl) {
+    if (!s || !u) return fail(DQ_ERR_ARG, "null argument");
+    if (int rc = check_qubit(s, target)) return rc;
+    if (n_ctrl < 0 || n_ctrl > 4) return fail(DQ_ERR_ARG, "at most 4 controls are supported");
+    HostOp op;
+    if (n_ctrl == 0) {
+        op.k = 1;
+        op.q[0] = target;
+        classify(op, u);
+        return enqueue(s, std::move(op));
+    }
+    op.kind = K_CTRL1;
+    op.k = n_ctrl + 1;
+    uint64_t seen = uint64_t(1) << target;
+    for (int c = 0; c < n_ctrl; ++c) {
+        if (int rc = check_qubit(s, ctrl_wires[c])) return rc;
+        const uint64_t m = uint64_t(1) << ctrl_wires[c];
+        if (seen & m) return fail(DQ_ERR_ARG, "control and target qubits must be distinct");
+        seen |= m;
+        op.q[c] = ctrl_wires[c];
+        op.cmask |= m;
+        if (ctrl_vals && ctrl_vals[c]) op.cval |= m;
+    }
+    op.q[n_ctrl] = target;
+    op.mat.assign(u, u + 8);
+    return enqueue(s, std::move(op));
+}
+
+int dq_apply_nq(dq_state *s, const double *u, const int *qubits, int k) {
+    if (!s || !u || !qubits) return fail(DQ_ERR_ARG, "null argument");
+    if (k < 1 || k > 5) return fail(DQ_ERR_ARG, "gate arity must be 1..5 (got " + std::to_string(k) + ")");
+    if (k > s->n) return fail(DQ_ERR_ARG, "gate arity exceeds the number of qubits");
+    HostOp op;
+    op.k = k;
+    uint64_t seen = 0;
+    for (int j = 0; j < k; ++j) {
+        if (int rc = check_qubit(s, qubits[j])) return rc;
+        const uint64_t m = uint64_t(1) << qubits[j];
+        if (seen & m) return fail(DQ_ERR_ARG, "gate qubits must be distinct");
+        seen |= m;
+        op.q[j] = qubits[j];
+    }
+    classify(op, u);
+    return enqueue(s, std::move(op));
+}
+
+int dq_collapse(dq_state *s, int qubit, int outcome, double norm) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (int rc = check_qubit(s, qubit)) return rc;
+    if (outcome != 0 && outcome != 1) return fail(DQ_ERR_ARG, "outcome must be 0 or 1");
+    HostOp op;
+    op.kind = K_COLLAPSE;
+    op.k = 1;
+    op.q[0] = qubit;
+    op.outcome = outcome;
+    op.norm = norm;
+    s->queue.push_back(std::move(op));
+    if (s->queue.size() >= 4096) return flush(s);
+    return DQ_OK;
+}
+
+int dq_prob_one(dq_state *s, int qubit, double *p1) {
+    if (!s || !p1) return fail(DQ_ERR_ARG, "null argument");
+    if (int rc = check_qubit(s, qubit)) return rc;
+    if (int rc = flush(s)) return rc;
+    const uint64_t count = amp_count(s) / 2;
+    const int blocks = std::min<int>(kReduceBlocks, grid_for(count));
+    if (s->dtype == DQ_C128)
+        hipLaunchKernelGGL(k_prob_one<double>, dim3(blocks), dim3(kThreads), 0, s->stream,
+                           static_cast<const cplx<double> *>(s->amps), s->n, qubit, s->d_partials);
+    else
+        hipLaunchKernelGGL(k_prob_one<float>, dim3(blocks), dim3(kThreads), 0, s->stream,
+                           static_cast<const cplx<float> *>(s->amps), s->n, qubit, s->d_partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, s->stream, s->d_partials, blocks, 1,
+                       s->d_result);
+    HIP_TRY(hipGetLastError());
+    s->launches += 2;
+    if (int rc = fetch_result(s, 1)) return rc;
+    *p1 = s->h_result[0];
+    return DQ_OK;
+}
+
+int dq_marginal_probs(dq_state *s, const int *qubits, int k, double *out) {
+    if (!s || !out || (k > 0 && !qubits)) return fail(DQ_ERR_ARG, "null argument");
+    if (k < 0 || k > s->n) return fail(DQ_ERR_ARG, "number of qubits out of range");
+    if (k > 30) return fail(DQ_ERR_ARG, "marginal over more than 30 qubits is not supported");
+    uint64_t seen = 0;
+    for (int j = 0; j < k; ++j) {
+        if (int rc = check_qubit(s, qubits[j])) return rc;
+        const uint64_t m = uint64_t(1) << qubits[j];
+        if (seen & m) return fail(DQ_ERR_ARG, "qubits must be distinct");
+        seen |= m;
+    }
+    if (int rc = flush(s)) return rc;
+    const int n = s->n;
+    const uint64_t bins = uint64_t(1) << k;
+    std::vector<int> meta(64 + 64);
+    for (int j = 0; j < k; ++j) meta[j] = qubits[j];
+    int nrest = 0;
+    for (int q = 0; q < n; ++q)
+        if (!((seen >> q) & 1)) meta[64 + nrest++] = q;
+    // device buffers: meta (ints) + output bins (+ partials)
+    const uint64_t terms = uint64_t(1) << (n - k);
+    int chunks = 1;
+    if (terms > 64) {
+        // enough blocks to fill the device, at least 256 terms per block
+        uint64_t want = std::max<uint64_t>(1, 4096 / bins);
+        while (uint64_t(chunks) * 2 <= want && terms / (uint64_t(chunks) * 2) >= 256) chunks *= 2;
+    }
+    const size_t meta_b = 128 * sizeof(int);
+    const size_t out_b = bins * sizeof(double);
+    const size_t part_b = (terms > 64) ? bins * chunks * sizeof(double) : 0;
+    void *dbuf = nullptr;
+    if (hipMalloc(&dbuf, meta_b + out_b + part_b) != hipSuccess)
+        return fail(DQ_ERR_OOM, "hipMalloc for marginal probabilities failed");
+    int *d_meta = static_cast<int *>(dbuf);
+    double *d_out = reinterpret_cast<double *>(static_cast<unsigned char *>(dbuf) + meta_b);
+    double *d_part = reinterpret_cast<double *>(static_cast<unsigned char *>(dbuf) + meta_b + out_b);
+    hipError_t e = hipMemcpyAsync(d_meta, meta.data(), meta_b, hipMemcpyHostToDevice, s->stream);
+    bool identity = (k == n);
+    for (int j = 0; j < k && identity; ++j) identity = (qubits[j] == n - 1 - j);
+    if (e == hipSuccess) {
+        if (k == n) {
+            if (s->dtype == DQ_C128)
+                hipLaunchKernelGGL(k_probs_perm<double>, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream,
+                                   static_cast<const cplx<double> *>(s->amps), n, d_meta, d_out, int(identity));
+            else
+                hipLaunchKernelGGL(k_probs_perm<float>, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream,
+                                   static_cast<const cplx<float> *>(s->amps), n, d_meta, d_out, int(identity));
+        } else if (terms <= 64) {
+            if (s->dtype == DQ_C128)
+                hipLaunchKernelGGL(k_marginal_seq<double>, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream,
+                                   static_cast<const cplx<double> *>(s->amps), n, d_meta, k, d_meta + 64, d_out);
+            else
+                hipLaunchKernelGGL(k_marginal_seq<float>, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream,
+                                   static_cast<const cplx<float> *>(s->amps), n, d_meta, k, d_meta + 64, d_out);
+        } else {
+            const unsigned nblk = unsigned(bins * chunks);
+            if (s->dtype == DQ_C128)
+                hipLaunchKernelGGL(k_marginal_tree<double>, dim3(nblk), dim3(kThreads), 0, s->stream,
+                                   static_cast<const cplx<double> *>(s->amps), n, d_meta, k, d_meta + 64, chunks,
+                                   d_part);
+            else
+                hipLaunchKernelGGL(k_marginal_tree<float>, dim3(nblk), dim3(kThreads), 0, s->stream,
+                                   static_cast<const cplx<float> *>(s->amps), n, d_meta, k, d_meta + 64, chunks,
+                                   d_part);
+            hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream, d_part, chunks,
+                               bins, d_out);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, out_b, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dbuf);
+    if (e != hipSuccess) return fail(DQ_ERR_DEVICE, std::string("marginal_probs: ") + hipGetErrorString(e));
+    s->launches += 2;
+    return DQ_OK;
+}
+
+int dq_inner(dq_state *s, const double *other, double *re, double *im) {
+    if (!s || !other || !re || !im) return fail(DQ_ERR_ARG, "null argument");
+    if (int rc = flush(s)) return rc;
+    const uint64_t count = amp_count(s);
+    void *tmp = nullptr;
+    if (hipMalloc(&tmp, 16 * count) != hipSuccess) return fail(DQ_ERR_OOM, "hipMalloc for inner product failed");
+    hipError_t e = hipMemcpyAsync(tmp, other, 16 * count, hipMemcpyHostToDevice, s->stream);
+    const int blocks = std::min<int>(kReduceBlocks, grid_for(count));
+    if (e == hipSuccess) {
+        if (s->dtype == DQ_C128)
+            hipLaunchKernelGGL(k_inner<double>, dim3(blocks), dim3(kThreads), 0, s->stream,
+                               static_cast<const cplx<double> *>(s->amps), static_cast<const cplx<double> *>(tmp),
+                               count, s->d_partials);
+        else
+            hipLaunchKernelGGL(k_inner<float>, dim3(blocks), dim3(kThreads), 0, s->stream,
+                               static_cast<const cplx<float> *>(s->amps), static_cast<const cplx<double> *>(tmp),
+                               count, s->d_partials);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, s->stream, s->d_partials, blocks, 2,
+                           s->d_result);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(DQ_ERR_DEVICE, std::string("inner: ") + hipGetErrorString(e));
+    if (int rc = fetch_result(s, 2)) return rc;
+    *re = s->h_result[0];
+    *im = s->h_result[1];
+    return DQ_OK;
+}
+
+int dq_copy_to_host(dq_state *s, double *dst) {
+    if (!s || !dst) return fail(DQ_ERR_ARG, "null argument");
+    if (int rc = flush(s)) return rc;
+    const uint64_t count = amp_count(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->dtype == DQ_C128) {
+        HIP_TRY(hipMemcpy(dst, s->amps, 16 * count, hipMemcpyDeviceToHost));
+        return DQ_OK;
+    }
+    void *tmp = nullptr;
+    if (hipMalloc(&tmp, 16 * count) != hipSuccess) return fail(DQ_ERR_OOM, "hipMalloc failed");
+    hipLaunchKernelGGL(k_to_f64<float>, dim3(grid_for(count)), dim3(kThreads), 0, s->stream,
+                       static_cast<cplx<double> *>(tmp), static_cast<const cplx<float> *>(s->amps), count);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipMemcpy(dst, tmp, 16 * count, hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(DQ_ERR_DEVICE, hipGetErrorString(e));
+    return DQ_OK;
+}
+
+int dq_synchronize(dq_state *s) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (int rc = flush(s)) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return DQ_OK;
+}
+
+int dq_set_options(dq_state *s, int fuse, int tile_bits) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (int rc = flush(s)) return rc;
+    const int max_bits = 13;
+    if (tile_bits != 0 && (tile_bits < 6 || tile_bits > max_bits))
+        return fail(DQ_ERR_ARG, "tile_bits out of range");
+    if (fuse < 0 || fuse > 2) return fail(DQ_ERR_ARG, "fusion level must be 0, 1 or 2");
+    s->fuse = fuse;
+    if (tile_bits) s->tile_bits = tile_bits;
+    return DQ_OK;
+}
+
+int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches, uint64_t *gates) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (sweeps) *sweeps = s->sweeps;
+    if (launches) *launches = s->launches;
+    if (gates) *gates = s->gates;
+    return DQ_OK;
+}
+
+}  // extern "C"

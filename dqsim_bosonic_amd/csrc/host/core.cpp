@@ -1,0 +1,518 @@
+// core.cpp -- the Python module `dqsim_bosonic_amd._core`.
+//
+// Host side of the statevector path, in C++ because the reference's host is
+// compiled code (Rust/PyO3).  It keeps the reference's Python surface:
+//   StatevectorSimulator(seed=None, profile=False).simulate(circuit)
+//       src/monolithic/statevector/simulator.rs:206-210, :325-379
+//   SimulationResult (statevector, num_qubits, classical_bits, profile,
+//       probabilities, counts, fidelity)          simulator.rs:94-192
+//   SimulationProfile (+ __repr__)                simulator.rs:52-88
+//   simulate_monolithic / simulate_monolithic_shots (state-vector mode)
+//       src/simulator.rs:13-45, :63-95; src/codecs.rs:19-28, :68-84
+// and the reference's instruction dispatcher (run_instruction,
+// simulator.rs:386-632).  All state mutation and reductions go through the
+// C ABI of include/dqsim_gpu.h (the HIP engine); measurement draws stay on
+// the host in the reference's ChaCha8 order.
+#include <pybind11/complex.h>
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <cctype>
+#include <chrono>
+#include <complex>
+#include <cstring>
+#include <functional>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <optional>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/dqsim_gpu.h"
+#include "circuit.hpp"
+#include "gates.hpp"
+#include "rng.hpp"
+
+namespace py = pybind11;
+using namespace dqh;
+
+namespace {
+
+[[noreturn]] void raise_dq(int rc, const char *what) {
+    std::string msg = std::string(what) + ": " + dq_last_error();
+    if (rc == DQ_ERR_ARG) throw py::value_error(msg);
+    if (rc == DQ_ERR_OOM) {
+        PyErr_SetString(PyExc_MemoryError, msg.c_str());
+        throw py::error_already_set();
+    }
+    throw std::runtime_error(msg);
+}
+
+inline void check(int rc, const char *what) {
+    if (rc != DQ_OK) raise_dq(rc, what);
+}
+
+// Rust's `{:?}` of a string: quoted with escapes.
+std::string rust_debug(const std::string &s) {
+    std::string o = "\"";
+    for (unsigned char c : s) {
+        switch (c) {
+        case '"': o += "\\\""; break;
+        case '\\': o += "\\\\"; break;
+        case '\n': o += "\\n"; break;
+        case '\r': o += "\\r"; break;
+        case '\t': o += "\\t"; break;
+        default:
+            if (c < 0x20 || c == 0x7f) {
+                char buf[16];
+                std::snprintf(buf, sizeof buf, "\\u{%x}", c);
+                o += buf;
+            } else {
+                o += char(c);
+            }
+        }
+    }
+    return o + "\"";
+}
+
+// RAII owner of a device state.
+struct DeviceState {
+    dq_state *s = nullptr;
+    explicit DeviceState(int n, int dtype = DQ_C128) { check(dq_state_create(n, dtype, &s), "statevector allocation"); }
+    ~DeviceState() { dq_state_destroy(s); }
+    DeviceState(const DeviceState &) = delete;
+    DeviceState &operator=(const DeviceState &) = delete;
+};
+
+struct SimulationProfile {
+    uint64_t oq_calls = 0, nq_calls = 0, mq_calls = 0;
+    double oq_time = 0, nq_time = 0, mq_time = 0, total_time = 0;
+
+    // simulator.rs:72-87
+    std::string repr() const {
+        const double total = std::max(total_time, 1e-9);
+        char buf[512];
+        std::snprintf(buf, sizeof buf,
+                      "SimulationProfile (total: %.2f ms)\n"
+                      "  apply_one_qubit : %4llu calls  %8.2f ms  (%.1f%%)\n"
+                      "  apply_n_qubit   : %4llu calls  %8.2f ms  (%.1f%%)\n"
+                      "  measure_qubit   : %4llu calls  %8.2f ms  (%.1f%%)",
+                      total_time * 1000.0, (unsigned long long)oq_calls, oq_time * 1000.0,
+                      100.0 * oq_time / total, (unsigned long long)nq_calls, nq_time * 1000.0,
+                      100.0 * nq_time / total, (unsigned long long)mq_calls, mq_time * 1000.0,
+                      100.0 * mq_time / total);
+        return buf;
+    }
+};
+
+using Clock = std::chrono::steady_clock;
+
+std::vector<int> qubits_list(const std::optional<std::vector<int64_t>> &qubits, int n) {
+    std::vector<int> qs;
+    if (!qubits) {
+        for (int q = n - 1; q >= 0; --q) qs.push_back(q);
+        return qs;
+    }
+    for (int64_t q : *qubits) {
+        if (q < 0 || q >= n) throw py::value_error("qubit index " + std::to_string(q) + " out of range");
+        qs.push_back(int(q));
+    }
+    return qs;
+}
+
+class SimulationResult {
+   public:
+    SimulationResult(std::shared_ptr<DeviceState> st, int n, std::map<uint64_t, int> cbits, py::object prof)
+        : st_(std::move(st)), n_(n), cbits_(std::move(cbits)), prof_(std::move(prof)) {}
+
+    int num_qubits() const { return n_; }
+
+    // simulator.rs:122-125: a fresh complex128 copy on every access.
+    py::array_t<std::complex<double>> statevector() const {
+        py::array_t<std::complex<double>> out(size_t(1) << n_);
+        auto *p = reinterpret_cast<double *>(out.mutable_data());
+        {
+            py::gil_scoped_release nogil;
+            check(dq_copy_to_host(st_->s, p), "statevector copy");
+        }
+        return out;
+    }
+
+    py::dict classical_bits() const {
+        py::dict d;
+        for (const auto &kv : cbits_) d[py::int_(kv.first)] = py::int_(kv.second);
+        return d;
+    }
+
+    py::object profile() const { return prof_; }
+
+    std::vector<double> marginal(const std::vector<int> &qs) const {
+        std::vector<double> probs(size_t(1) << qs.size());
+        check(dq_marginal_probs(st_->s, qs.data(), int(qs.size()), probs.data()), "marginal probabilities");
+        return probs;
+    }
+
+    // simulator.rs:148-159
+    py::dict probabilities(const std::optional<std::vector<int64_t>> &qubits) const {
+        const std::vector<int> qs = qubits_list(qubits, n_);
+        const std::vector<double> probs = marginal(qs);
+        py::dict d;
+        for (size_t j = 0; j < probs.size(); ++j)
+            if (probs[j] > 0.0) d[py::int_(j)] = py::float_(probs[j]);
+        return d;
+    }
+
+    // simulator.rs:162-181 over engine::sample_counts (src/engine.rs:398-433)
+    py::dict counts(uint64_t shots, const std::optional<std::vector<int64_t>> &qubits,
+                    const std::optional<uint64_t> &seed) const {
+        const std::vector<int> qs = qubits_list(qubits, n_);
+        ChaCha8Rng rng = seed ? ChaCha8Rng::seed_from_u64(*seed) : ChaCha8Rng::from_entropy();
+        const std::vector<double> probs = marginal(qs);
+        const size_t dim = probs.size();
+        const int k = int(qs.size());
+        std::vector<double> cdf(dim);
+        double acc = 0.0;
+        for (size_t i = 0; i < dim; ++i) {
+            acc += probs[i];
+            cdf[i] = acc;
+        }
+        cdf[dim - 1] = 1.0;
+        std::unordered_map<uint64_t, uint64_t> hist;
+        for (uint64_t s = 0; s < shots; ++s) {
+            const double r = rng.gen_f64();
+            size_t idx = size_t(std::lower_bound(cdf.begin(), cdf.end(), r) - cdf.begin());
+            if (idx > dim - 1) idx = dim - 1;
+            hist[idx]++;
+        }
+        py::dict d;
+        for (const auto &kv : hist) {
+            std::string bits;
+            uint64_t v = kv.first;
+            do {
+                bits.insert(bits.begin(), char('0' + (v & 1)));
+                v >>= 1;
+            } while (v);
+            if (int(bits.size()) < k) bits.insert(bits.begin(), size_t(k) - bits.size(), '0');
+            d[py::str(bits)] = py::int_(kv.second);
+        }
+        return d;
+    }
+
+    // simulator.rs:184-191
+    double fidelity(py::array_t<std::complex<double>, py::array::c_style> other) const {
+        if (other.ndim() != 1) throw py::value_error("expected a one-dimensional array");
+        if (uint64_t(other.shape(0)) != (uint64_t(1) << n_)) return 0.0;
+        double re = 0, im = 0;
+        check(dq_inner(st_->s, reinterpret_cast<const double *>(other.data()), &re, &im), "fidelity");
+        return re * re + im * im;
+    }
+
+    dq_state *handle() const { return st_->s; }
+
+   private:
+    std::shared_ptr<DeviceState> st_;
+    int n_;
+    std::map<uint64_t, int> cbits_;
+    py::object prof_;
+};
+
+// Generic `gate` names accepted by the statevector dispatcher
+// (simulator.rs:571-600) -> the gate library's matrix name.
+const std::unordered_map<std::string, std::string> &generic_gates() {
+    static const std::unordered_map<std::string, std::string> m = {
+        {"remote_link_phi_plus", "remote_link_phi_plus"}, {"remote_link_psi_minus", "remote_link_psi_minus"},
+        {"remote_link_psi_plus", "remote_link_psi_plus"}, {"nonlocal_cz", "cz"},
+        {"remote_cz", "cz"},                              {"remote_cx", "cx"},
+        {"remote_epr", "remote_link_phi_plus"}};
+    return m;
+}
+
+struct NotImplemented : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+class Runner {
+   public:
+    Runner(dq_state *s, int n, uint64_t seed, bool profile)
+        : s_(s), n_(n), rng_(ChaCha8Rng::seed_from_u64(seed)), profile_(profile) {}
+
+    std::map<uint64_t, int> cbits;
+    SimulationProfile acc;
+
+    // run_instruction, simulator.rs:386-632
+    void run(const Instr &in) {
+        const std::string &k = in.kind;
+        if (k == "id" || k == "u0" || k == "barrier" || k == "classical") return;
+        if (k == "measure") {
+            cbits[in.cbit] = measure(in.qubits[0]);
+            return;
+        }
+        if (k == "reset") {
+            if (measure(in.qubits[0]) == 1) apply_x(int(qubit(in.qubits[0])));
+            return;
+        }
+        if (k == "conditional") {
+            uint64_t actual = 0;
+            for (uint64_t b = 0; b < in.creg_size; ++b) {
+                auto it = cbits.find(in.creg_base + b);
+                const uint64_t v = it == cbits.end() ? 0 : uint64_t(int64_t(it->second));
+                actual |= v << (b & 63);
+            }
+            if (actual == in.creg_value) run(*in.op);
+            return;
+        }
+        Mat m;
+        if (k == "gate") {
+            std::string name = in.name;
+            for (auto &c : name) c = char(std::tolower(static_cast<unsigned char>(c)));
+            if (name == "remote_barrier" || name == "remote_cu1") return;
+            auto it = generic_gates().find(name);
+            if (it == generic_gates().end())
+                throw NotImplemented("Unsupported generic gate: " + rust_debug(name) +
+                                     ". Decompose it before simulating.");
+            gate_matrix(it->second, {}, m);
+            apply_n(m, in.qubits);
+            return;
+        }
+        if (!gate_matrix(k, in.params, m)) throw NotImplemented("unsupported instruction kind " + rust_debug(k));
+        if (in.qubits.size() == 1) {
+            apply_1(m, int(qubit(in.qubits[0])));
+        } else {
+            apply_n(m, in.qubits);
+        }
+    }
+
+   private:
+    dq_state *s_;
+    int n_;
+    ChaCha8Rng rng_;
+    bool profile_;
+
+    uint64_t qubit(uint64_t q) const {
+        if (q >= uint64_t(n_))
+            throw py::value_error("qubit index " + std::to_string(q) + " out of range for " + std::to_string(n_) +
+                                  "-qubit circuit");
+        return q;
+    }
+
+    template <typename F>
+    void timed(uint64_t &calls, double &secs, F &&f) {
+        if (!profile_) {
+            f();
+            return;
+        }
+        const auto t0 = Clock::now();
+        f();
+        check(dq_synchronize(s_), "synchronize");
+        secs += std::chrono::duration<double>(Clock::now() - t0).count();
+        calls++;
+    }
+
+    // do_oq, simulator.rs:890-907
+    void apply_1(const Mat &m, int q) {
+        timed(acc.oq_calls, acc.oq_time, [&] { check(dq_apply_1q(s_, m.v.data(), q, nullptr, nullptr, 0), "apply_1q"); });
+    }
+    void apply_x(int q) {
+        Mat x;
+        gate_matrix("x", {}, x);
+        apply_1(x, q);
+    }
+    // do_nq, simulator.rs:909-926
+    void apply_n(const Mat &m, const std::vector<uint64_t> &qubits) {
+        if (int(qubits.size()) != m.k)
+            throw py::value_error("gate expects " + std::to_string(m.k) + " qubits, got " +
+                                  std::to_string(qubits.size()));
+        int qs[5];
+        for (int j = 0; j < m.k; ++j) qs[j] = int(qubit(qubits[j]));
+        timed(acc.nq_calls, acc.nq_time, [&] { check(dq_apply_nq(s_, m.v.data(), qs, m.k), "apply_nq"); });
+    }
+    // do_mq + engine::measure_qubit, simulator.rs:928-946, src/engine.rs:193-257
+    int measure(uint64_t q64) {
+        const int q = int(qubit(q64));
+        int outcome = 0;
+        timed(acc.mq_calls, acc.mq_time, [&] {
+            double p1 = 0.0;
+            check(dq_prob_one(s_, q, &p1), "prob_one");
+            outcome = rng_.gen_f64() < p1 ? 1 : 0;
+            const double norm =
+                outcome ? std::max(std::sqrt(p1), 1e-15) : std::max(std::sqrt(std::max(1.0 - p1, 0.0)), 1e-15);
+            check(dq_collapse(s_, q, outcome, norm), "collapse");
+        });
+        return outcome;
+    }
+};
+
+class StatevectorSimulator {
+   public:
+    StatevectorSimulator(std::optional<uint64_t> seed, bool profile, int fusion = 2, int tile_bits = 0)
+        : seed_(seed), profile_(profile), fusion_(fusion), tile_bits_(tile_bits) {}
+
+    // simulator.rs:325-379
+    SimulationResult simulate(py::object circuit) const {
+        const std::string text = py::str(circuit.attr("model_dump_json")());
+        Circuit c;
+        try {
+            c = Decoder().decode(text);
+        } catch (const JsonError &e) {
+            throw py::value_error(std::string("Circuit JSON parse error: ") + e.what());
+        }
+        const uint64_t n64 = c.num_qubits();
+        if (n64 > 40) throw py::value_error("circuit has " + std::to_string(n64) + " qubits; at most 40 are supported");
+        const int n = int(n64);
+        auto st = std::make_shared<DeviceState>(n);
+        check(dq_set_options(st->s, fusion_, tile_bits_), "options");
+        const uint64_t seed = seed_ ? *seed_ : ChaCha8Rng::entropy_u64();
+        Runner run(st->s, n, seed, profile_);
+        const auto t0 = Clock::now();
+        try {
+            for (const Instr &in : c.instructions) run.run(in);
+            check(dq_synchronize(st->s), "synchronize");
+        } catch (const NotImplemented &e) {
+            PyErr_SetString(PyExc_NotImplementedError, e.what());
+            throw py::error_already_set();
+        }
+        run.acc.total_time = std::chrono::duration<double>(Clock::now() - t0).count();
+        py::object prof = py::none();
+        if (profile_) prof = py::cast(run.acc);
+        return SimulationResult(st, n, run.cbits, prof);
+    }
+
+    std::optional<uint64_t> seed_;
+    bool profile_;
+    int fusion_;
+    int tile_bits_;
+};
+
+enum class Mode { StateVector, Mps, Stabilizer };
+
+// codecs.rs:19-28
+Mode parse_mode(const std::string &mode) {
+    std::string m;
+    size_t a = 0, b = mode.size();
+    while (a < b && std::isspace(static_cast<unsigned char>(mode[a]))) ++a;
+    while (b > a && std::isspace(static_cast<unsigned char>(mode[b - 1]))) --b;
+    for (size_t i = a; i < b; ++i) {
+        char c = char(std::tolower(static_cast<unsigned char>(mode[i])));
+        m += (c == '-') ? '_' : c;
+    }
+    if (m == "state_vector" || m == "statevector" || m == "sv") return Mode::StateVector;
+    if (m == "mps" || m == "matrix_product_state") return Mode::Mps;
+    if (m == "stabilizer" || m == "stab") return Mode::Stabilizer;
+    throw py::value_error("Unsupported monolithic simulation mode " + rust_debug(m) +
+                          "; expected 'state_vector' or 'mps'");
+}
+
+// codecs.rs:68-84
+void reject_options(const py::kwargs &options, const std::string &mode) {
+    if (options.size() == 0) return;
+    std::string keys = "[";
+    bool first = true;
+    for (auto item : options) {
+        if (!first) keys += ", ";
+        first = false;
+        keys += rust_debug(py::str(item.first));
+    }
+    keys += "]";
+    throw py::type_error("Options " + keys + " are not supported for monolithic mode " + rust_debug(mode));
+}
+
+[[noreturn]] void other_engine(const char *what) {
+    PyErr_SetString(PyExc_NotImplementedError, what);
+    throw py::error_already_set();
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_core, m) {
+    m.doc() = "MI355X statevector engine: the reference's dqsim statevector API over HIP kernels";
+
+    py::class_<SimulationProfile>(m, "SimulationProfile")
+        .def_readonly("apply_one_qubit_calls", &SimulationProfile::oq_calls)
+        .def_readonly("apply_one_qubit_time", &SimulationProfile::oq_time)
+        .def_readonly("apply_n_qubit_calls", &SimulationProfile::nq_calls)
+        .def_readonly("apply_n_qubit_time", &SimulationProfile::nq_time)
+        .def_readonly("measure_qubit_calls", &SimulationProfile::mq_calls)
+        .def_readonly("measure_qubit_time", &SimulationProfile::mq_time)
+        .def_readonly("total_time", &SimulationProfile::total_time)
+        .def("__repr__", &SimulationProfile::repr);
+
+    py::class_<SimulationResult>(m, "SimulationResult")
+        .def_property_readonly("statevector", &SimulationResult::statevector)
+        .def_property_readonly("num_qubits", &SimulationResult::num_qubits)
+        .def_property_readonly("classical_bits", &SimulationResult::classical_bits)
+        .def_property_readonly("profile", &SimulationResult::profile)
+        .def("probabilities", &SimulationResult::probabilities, py::arg("qubits") = py::none())
+        .def("counts", &SimulationResult::counts, py::arg("shots") = 1000, py::arg("qubits") = py::none(),
+             py::arg("seed") = py::none())
+        .def("fidelity", &SimulationResult::fidelity, py::arg("other"));
+
+    py::class_<StatevectorSimulator>(m, "StatevectorSimulator")
+        .def(py::init<std::optional<uint64_t>, bool, int, int>(), py::arg("seed") = py::none(),
+             py::arg("profile") = false, py::kw_only(), py::arg("fusion") = 2, py::arg("tile_bits") = 0)
+        .def("simulate", &StatevectorSimulator::simulate, py::arg("circuit"));
+
+    m.def(
+        "simulate_monolithic",
+        [](py::object circuit, const std::string &mode, std::optional<uint64_t> seed, bool profile,
+           py::kwargs options) -> py::object {
+            switch (parse_mode(mode)) {
+            case Mode::StateVector:
+                reject_options(options, mode);
+                return py::cast(StatevectorSimulator(seed, profile).simulate(circuit));
+            case Mode::Mps:
+                other_engine("mode 'mps' is a different algorithm and is not provided by this statevector engine");
+            case Mode::Stabilizer:
+                other_engine(
+                    "Full statevector extraction ('simulate_monolithic') is not supported for stabilizer "
+                    "simulation. Please use shot-based sampling via 'simulate_monolithic_shots' instead.");
+            }
+            return py::none();
+        },
+        py::arg("circuit"), py::arg("mode") = "state_vector", py::arg("seed") = py::none(),
+        py::arg("profile") = false);
+
+    // Host-logic hooks used by the CPU test-suite (no device needed).
+    m.def("_gate_matrix", [](const std::string &name, std::vector<double> params) {
+        Mat mm;
+        if (!gate_matrix(name, params, mm)) throw py::key_error(name);
+        const size_t d = size_t(1) << mm.k;
+        py::array_t<std::complex<double>> out({d, d});
+        std::memcpy(out.mutable_data(), mm.v.data(), mm.v.size() * sizeof(double));
+        return out;
+    });
+    m.def("_rng_stream", [](uint64_t seed, size_t count) {
+        ChaCha8Rng r = ChaCha8Rng::seed_from_u64(seed);
+        std::vector<uint64_t> v(count);
+        for (auto &x : v) x = r.next_u64();
+        return v;
+    });
+    m.def("_decode", [](const std::string &text) {
+        Circuit c;
+        try {
+            c = Decoder().decode(text);
+        } catch (const JsonError &e) {
+            throw py::value_error(std::string("Circuit JSON parse error: ") + e.what());
+        }
+        py::list out;
+        std::function<py::dict(const Instr &)> conv = [&](const Instr &in) {
+            py::dict d;
+            d["kind"] = in.kind;
+            d["qubits"] = in.qubits;
+            d["params"] = in.params;
+            d["name"] = in.name;
+            d["cbit"] = in.cbit;
+            d["condition"] = py::make_tuple(in.creg_base, in.creg_size, in.creg_value);
+            if (in.op) d["op"] = conv(*in.op);
+            return d;
+        };
+        for (const Instr &in : c.instructions) out.append(conv(in));
+        return py::make_tuple(c.num_qubits(), c.num_cbits(), out);
+    });
+    m.def("_parse_mode", [](const std::string &mode) { return int(parse_mode(mode)); });
+}

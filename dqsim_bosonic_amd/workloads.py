@@ -1,0 +1,129 @@
+"""Synthetic circuit families used by the benchmark and the parity tests.
+
+Each generator returns a ``circuit.Circuit`` in the reference's wire schema
+(src/types.rs:60-135), drawn from numpy's PCG64 ``default_rng(seed)``.
+
+* ``brickwork_u3``: depth-d brickwork, u3 on every qubit per layer, then cx on
+  (2i, 2i+1) on even layers and (2i+1, 2i+2) on odd layers (20 qubits x 20
+  layers is the reference's CPU-runnable plain workload).
+* ``random_blocks``: the at-scale plain workload: per layer one random gate
+  from {h, s, t, sx, rx, ry, rz} on every qubit, {cz, rzz, cp} on a random
+  perfect matching, and 8 remote_link_psi_plus + 8 nonlocal_cz (generic
+  ``gate`` kind) between the two halves of the register; after every 10th
+  layer, measure two random qubits, reset them, and apply one x conditioned
+  on the first measured bit.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import circuit as C
+
+TWO_PI = 2.0 * math.pi
+
+
+def brickwork_u3(n: int = 20, depth: int = 20, seed: int = 42) -> C.Circuit:
+    rng = np.random.default_rng(seed)
+    ins = []
+    for layer in range(depth):
+        ang = rng.uniform(0.0, TWO_PI, size=(n, 3))
+        for q in range(n):
+            ins.append(C.U3Instruction(qubit=q, theta=float(ang[q, 0]), phi=float(ang[q, 1]),
+                                       lam=float(ang[q, 2])))
+        start = 0 if layer % 2 == 0 else 1
+        for a in range(start, n - 1, 2):
+            ins.append(C.CxInstruction(control=a, target=a + 1))
+    return C.circuit(n, ins)
+
+
+_ONE_Q = ("h", "s", "t", "sx", "rx", "ry", "rz")
+_PARAM = {"rx": "theta", "ry": "theta", "rz": "phi"}
+_TWO_Q = ("cz", "rzz", "cp")
+
+
+def random_blocks(n: int = 30, depth: int = 40, seed: int = 42, remote_per_kind: int = 8,
+                  measure_every: int = 10) -> C.Circuit:
+    rng = np.random.default_rng(seed)
+    half = n // 2
+    block_a = list(range(half))
+    block_b = list(range(half, n))
+    ins = []
+    n_meas = 0
+    for layer in range(depth):
+        for q in range(n):
+            g = _ONE_Q[int(rng.integers(len(_ONE_Q)))]
+            if g in _PARAM:
+                ins.append(C.instruction(g, qubit=q, **{_PARAM[g]: float(rng.uniform(0.0, TWO_PI))}))
+            else:
+                ins.append(C.instruction(g, qubit=q))
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            a, b = int(perm[i]), int(perm[i + 1])
+            g = _TWO_Q[int(rng.integers(len(_TWO_Q)))]
+            if g == "cz":
+                ins.append(C.CzInstruction(control=a, target=b))
+            elif g == "rzz":
+                ins.append(C.RzzInstruction(a=a, b=b, theta=float(rng.uniform(0.0, TWO_PI))))
+            else:
+                ins.append(C.CpInstruction(control=a, target=b, lam=float(rng.uniform(0.0, TWO_PI))))
+        for name in ("remote_link_psi_plus", "nonlocal_cz"):
+            for _ in range(remote_per_kind):
+                a = int(rng.choice(block_a))
+                b = int(rng.choice(block_b))
+                ins.append(C.GateInstruction(name=name, params=[], qubits=[a, b]))
+        if measure_every and (layer + 1) % measure_every == 0:
+            q1, q2, q3 = (int(x) for x in rng.choice(n, size=3, replace=False))
+            c1, c2 = n_meas, n_meas + 1
+            n_meas += 2
+            ins.append(C.MeasureInstruction(qubit=q1, cbit=c1))
+            ins.append(C.MeasureInstruction(qubit=q2, cbit=c2))
+            ins.append(C.ResetInstruction(qubit=q1))
+            ins.append(C.ResetInstruction(qubit=q2))
+            ins.append(C.ConditionalInstruction(
+                condition=C.Condition(creg_base=c1, creg_size=1, creg_value=1),
+                op=C.XInstruction(qubit=q3)))
+    return C.circuit(n, ins, n_cbits=max(n_meas, 1))
+
+
+def gate_count(circ: C.Circuit) -> int:
+    """Unitary gates in a circuit (measure/reset/barrier excluded; a
+    conditional counts as its gate)."""
+    cnt = 0
+    for i in circ.instructions:
+        if i.kind in ("measure", "reset", "barrier", "classical", "id", "u0"):
+            continue
+        cnt += 1
+    return cnt
+
+
+def random_mixed(n: int, count: int, seed: int, kinds=None) -> C.Circuit:
+    """Random circuit over every gate kind of the reference's dispatcher
+    (simulator.rs:394-600), used by the parity tests."""
+    rng = np.random.default_rng(seed)
+    kinds = kinds or [k for k in C._KIND_FIELDS if k not in ("measure", "reset", "conditional", "barrier",
+                                                            "classical", "gate")]
+    generic = ["remote_link_phi_plus", "remote_link_psi_minus", "remote_link_psi_plus", "nonlocal_cz",
+               "remote_cz", "remote_cx", "remote_epr", "remote_barrier", "remote_cu1"]
+    ins = []
+    qubit_fields = {"qubit", "control", "target", "a", "b", "control1", "control2", "control3", "control4",
+                    "target1", "target2"}
+    for _ in range(count):
+        if rng.random() < 0.1:
+            name = generic[int(rng.integers(len(generic)))]
+            qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+            ins.append(C.GateInstruction(name=name, params=[], qubits=qs))
+            continue
+        k = kinds[int(rng.integers(len(kinds)))]
+        fields = C._KIND_FIELDS[k]
+        qf = [f for f in fields if f in qubit_fields]
+        if len(qf) > n:
+            continue
+        qs = [int(x) for x in rng.choice(n, size=len(qf), replace=False)]
+        kw = dict(zip(qf, qs))
+        for f in fields:
+            if f not in qubit_fields:
+                kw[f] = float(rng.uniform(-TWO_PI, TWO_PI))
+        ins.append(C.instruction(k, **kw))
+    return C.circuit(n, ins)

@@ -1,0 +1,116 @@
+/*
+ * dqsim_gpu.h -- C ABI of the MI355X (gfx950) statevector engine.
+ *
+ * These entry points replace the Rayon kernels of the reference's
+ * src/engine.rs one for one; a Rust/PyO3 host binds them through an
+ * `extern "C"` block (see INTEGRATION.md for the binding).  Plain pointers
+ * and sizes only; no torch or HIP types cross the boundary.
+ *
+ * State layout in HBM: one contiguous array of 2^n amplitudes, interleaved
+ * (re, im), amplitude index bit q <-> qubit q (qubit 0 = LSB), exactly the
+ * reference's Vec<Complex64> layout (src/engine.rs:34,113).  Multi-qubit
+ * matrices are row-major 2^k x 2^k, (re, im) interleaved doubles, with
+ * qubits[0] the most significant bit of the row/column index
+ * (src/engine.rs:104-106,116-125).
+ *
+ * Execution model: gate applications are queued on the state's stream and
+ * may be fused (several gates applied in one HBM sweep, in program order, with
+ * each gate's arithmetic unchanged).  Every call that returns data to the host
+ * (dq_prob_one, dq_marginal_probs, dq_inner, dq_expect_*, dq_copy_to_host)
+ * first flushes the queue and synchronises.  A handle is single-stream and not
+ * re-entrant: calls on one handle must not overlap.
+ *
+ * Return codes: 0 ok, < 0 error; dq_last_error() gives a thread-local message.
+ */
+#ifndef DQSIM_GPU_H
+#define DQSIM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dq_state dq_state;
+
+typedef enum { DQ_C128 = 0, DQ_C64 = 1 } dq_dtype;
+
+enum {
+    DQ_OK = 0,
+    DQ_ERR_ARG = -1,     /* bad qubit / arity / size: host raises ValueError */
+    DQ_ERR_OOM = -2,     /* device allocation failed: host raises MemoryError */
+    DQ_ERR_DEVICE = -3,  /* HIP runtime error / no device: RuntimeError */
+};
+
+/* Allocates 2^n amplitudes on the current device and sets |0...0>.
+ * Replaces `vec![C::new(0.0,0.0); 1<<n]; state[0]=1`
+ * (src/monolithic/statevector/simulator.rs:338-339). */
+int dq_state_create(int n_qubits, int dtype, dq_state **out);
+void dq_state_destroy(dq_state *s);
+int dq_state_num_qubits(const dq_state *s);
+/* Overwrites the whole state from host memory (2^n (re,im) doubles). */
+int dq_state_set(dq_state *s, const double *src);
+/* Resets the state to |0...0>. */
+int dq_state_reset(dq_state *s);
+
+/* 2x2 gate u (row-major, (re,im)) on `target`, optionally restricted to the
+ * subspace where every ctrl_wires[c] has value ctrl_vals[c].
+ * Replaces engine::apply_one_qubit (src/engine.rs:24-101). */
+int dq_apply_1q(dq_state *s, const double u[8], int target, const int *ctrl_wires,
+                const unsigned char *ctrl_vals, int n_ctrl);
+
+/* Dense 2^k x 2^k gate on qubits[0..k) (qubits[0] = MSB), 1 <= k <= 5.
+ * Replaces engine::apply_n_qubit (src/engine.rs:109-181): each output
+ * amplitude is acc = 0; acc += U[r][c] * v[c] for c ascending. */
+int dq_apply_nq(dq_state *s, const double *u, const int *qubits, int k);
+
+/* p1 = sum over amplitudes with bit `qubit` set of |psi_i|^2 (FP64).
+ * Synchronous.  First half of engine::measure_qubit (src/engine.rs:197-208);
+ * the random draw stays on the host. */
+int dq_prob_one(dq_state *s, int qubit, double *p1);
+
+/* Zeroes the half with bit `qubit` != outcome and divides the kept half by
+ * `norm` (IEEE division).  Second half of engine::measure_qubit
+ * (src/engine.rs:212-254); the host computes norm = max(sqrt(p1),1e-15) or
+ * max(sqrt(max(1-p1,0)),1e-15). */
+int dq_collapse(dq_state *s, int qubit, int outcome, double norm);
+
+/* out[j] = sum_{i : bits(i, qubits) == j} |psi_i|^2, qubits[0] = MSB of j,
+ * 0 <= k <= n, out has 2^k doubles (host).  Replaces engine::marginal_probs
+ * (src/engine.rs:368-390) behind SimulationResult.probabilities/counts. */
+int dq_marginal_probs(dq_state *s, const int *qubits, int k, double *out);
+
+/* sum_i conj(psi_i) * other_i with `other` a host array of 2^n (re,im)
+ * doubles.  Replaces the fold in SimulationResult.fidelity
+ * (src/monolithic/statevector/simulator.rs:184-191). */
+int dq_inner(dq_state *s, const double *other, double *re, double *im);
+
+/* Copies the state to host memory (2^n (re,im) doubles; for DQ_C64 states the
+ * values are widened).  Backs SimulationResult.statevector
+ * (src/monolithic/statevector/simulator.rs:122-125). */
+int dq_copy_to_host(dq_state *s, double *dst);
+
+/* Flushes queued gates and waits for the device. */
+int dq_synchronize(dq_state *s);
+
+/* Execution options (no reference counterpart; tuning only).
+ *   fuse = 0: every gate is its own HBM sweep;
+ *   fuse = 1: gates are grouped into LDS-tiled sweeps in strict program
+ *             order (bit-identical to the reference's arithmetic);
+ *   fuse = 2: as 1, but a sweep may also take later gates on qubits disjoint
+ *             from every gate it skipped (default; identical in exact
+ *             arithmetic, last-bit rounding may differ).
+ *   tile_bits = 0 keeps the current tile size (6..13 qubits per tile). */
+int dq_set_options(dq_state *s, int fuse, int tile_bits);
+
+/* Number of HBM sweeps and kernel launches issued so far on this handle. */
+int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches,
+                 uint64_t *gates);
+
+const char *dq_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DQSIM_GPU_H */

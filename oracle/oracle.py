@@ -1,0 +1,316 @@
+"""CPU oracle for the statevector hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg import this module, and only as the checker or the timed CPU baseline.
+The product (``dqsim_bosonic_amd``) never imports it.
+
+The numeric kernels live in ``dqsim_oracle.c`` (built into ``liboracle.so``
+by ``oracle/Makefile``); this module restates the instruction dispatcher of
+the reference around them, independently of the product's C++ decoder:
+
+* ``OracleSimulator.simulate`` restates ``StatevectorSimulator::simulate``
+  (src/monolithic/statevector/simulator.rs:325-379): |0..0> init,
+  ChaCha8Rng::seed_from_u64(seed), instruction loop.
+* ``_run`` restates ``run_instruction`` (simulator.rs:386-632), including the
+  generic ``gate`` names (:571-600), measurement (:603-606), conditional
+  (:609-618) and reset (:621-626).
+* ``probabilities`` / ``counts`` / ``fidelity`` restate
+  simulator.rs:148-191 over ``marginal_probs`` / ``sample_counts``
+  (src/engine.rs:368-433).
+* ``num_qubits`` restates src/types.rs:24-30.
+
+Parity status of each piece is listed in the test modules that pin it.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import subprocess
+from typing import Any
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> str:
+    """Compile liboracle.so with oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.or_apply_one_qubit.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, ip, ip, ctypes.c_int]
+        L.or_apply_n_qubit.argtypes = [dp, ctypes.c_int, dp, ip, ctypes.c_int]
+        L.or_apply_one_qubit_par.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int]
+        L.or_apply_n_qubit_par.argtypes = [dp, ctypes.c_int, dp, ip, ctypes.c_int]
+        L.or_prob_one.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+        L.or_prob_one.restype = ctypes.c_double
+        L.or_prob_one_par.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+        L.or_prob_one_par.restype = ctypes.c_double
+        L.or_collapse.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.or_collapse_par.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.or_marginal_probs.argtypes = [dp, ctypes.c_int, ip, ctypes.c_int, dp]
+        L.or_marginal_probs_literal.argtypes = [dp, ctypes.c_int, ip, ctypes.c_int, dp]
+        L.or_sample_counts.argtypes = [dp, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_uint32)]
+        L.or_inner.argtypes = [dp, dp, ctypes.c_uint64, dp, dp]
+        L.or_gate_matrix.argtypes = [ctypes.c_char_p, dp, ctypes.c_int, dp]
+        L.or_gate_matrix.restype = ctypes.c_int
+        L.or_rng_seed_from_u64.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        L.or_rng_next_u64.argtypes = [ctypes.c_void_p]
+        L.or_rng_next_u64.restype = ctypes.c_uint64
+        L.or_rng_next_u32.argtypes = [ctypes.c_void_p]
+        L.or_rng_next_u32.restype = ctypes.c_uint32
+        L.or_rng_gen_f64.argtypes = [ctypes.c_void_p]
+        L.or_rng_gen_f64.restype = ctypes.c_double
+        L.or_chacha_block.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint64,
+                                      ctypes.c_uint64, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_uint32)]
+        L.or_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Rng:
+    """ChaCha8Rng::seed_from_u64 + gen::<f64>() (restated in dqsim_oracle.c)."""
+
+    _SIZE = 8 * 4 + 8 + 64 * 4 + 4 + 4
+
+    def __init__(self, seed: int):
+        self._buf = ctypes.create_string_buffer(self._SIZE + 16)
+        lib().or_rng_seed_from_u64(self._buf, ctypes.c_uint64(seed & (2**64 - 1)))
+
+    def next_u64(self) -> int:
+        return int(lib().or_rng_next_u64(self._buf))
+
+    def next_u32(self) -> int:
+        return int(lib().or_rng_next_u32(self._buf))
+
+    def gen_f64(self) -> float:
+        return float(lib().or_rng_gen_f64(self._buf))
+
+
+# -- gates.rs ----------------------------------------------------------------
+
+_PARAM_FIELDS = {
+    "u3": ("theta", "phi", "lam"), "u": ("theta", "phi", "lam"), "u2": ("phi", "lam"),
+    "u1": ("lam",), "p": ("lam",), "rx": ("theta",), "ry": ("theta",), "rz": ("phi",),
+    "crx": ("theta",), "cry": ("theta",), "crz": ("lam",), "cu1": ("lam",), "cp": ("lam",),
+    "cu3": ("theta", "phi", "lam"), "cu": ("theta", "phi", "lam", "gamma"),
+    "rxx": ("theta",), "rzz": ("theta",),
+}
+_QUBIT_FIELDS = {
+    **{k: ("qubit",) for k in ("id", "u0", "x", "y", "z", "h", "s", "sdg", "t", "tdg", "sx",
+                               "sxdg", "u3", "u", "u2", "u1", "p", "rx", "ry", "rz")},
+    **{k: ("control", "target") for k in ("cx", "cz", "cy", "ch", "csx", "crx", "cry", "crz",
+                                           "cu1", "cp", "cu3", "cu")},
+    **{k: ("a", "b") for k in ("swap", "rxx", "rzz")},
+    "ccx": ("control1", "control2", "target"), "rccx": ("control1", "control2", "target"),
+    "cswap": ("control", "target1", "target2"),
+    **{k: ("control1", "control2", "control3", "target") for k in ("rc3x", "c3x", "c3sqrtx")},
+    "c4x": ("control1", "control2", "control3", "control4", "target"),
+}
+# generic `gate` names the statevector dispatcher accepts (simulator.rs:571-600)
+_GENERIC = {
+    "remote_link_phi_plus": "remote_link_phi_plus", "remote_link_psi_minus": "remote_link_psi_minus",
+    "remote_link_psi_plus": "remote_link_psi_plus", "nonlocal_cz": "cz", "remote_cz": "cz",
+    "remote_cx": "cx", "remote_epr": "remote_link_phi_plus",
+}
+
+
+def gate_matrix(name: str, params=()) -> np.ndarray:
+    """Gate matrix from the oracle's restatement of src/gates.rs."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros(2 * 32 * 32, dtype=np.float64)
+    k = lib().or_gate_matrix(name.encode(), _dp(p) if p.size else None, int(p.size), _dp(out))
+    if k < 0:
+        raise KeyError(name)
+    d = 1 << k
+    return out[: 2 * d * d].view(np.complex128).reshape(d, d).copy()
+
+
+# -- engine.rs ---------------------------------------------------------------
+
+def apply_one_qubit(state: np.ndarray, n: int, u: np.ndarray, target: int, controls=()):
+    u = np.ascontiguousarray(u, dtype=np.complex128)
+    w = np.array([c[0] for c in controls], dtype=np.int32)
+    v = np.array([int(c[1]) for c in controls], dtype=np.int32)
+    lib().or_apply_one_qubit(_dp(state.view(np.float64)), n, _dp(u.view(np.float64)), target,
+                             _ip(w), _ip(v), len(controls))
+
+
+def apply_n_qubit(state: np.ndarray, n: int, u: np.ndarray, qubits, parallel=False):
+    u = np.ascontiguousarray(u, dtype=np.complex128)
+    q = np.ascontiguousarray(qubits, dtype=np.int32)
+    f = lib().or_apply_n_qubit_par if parallel else lib().or_apply_n_qubit
+    f(_dp(state.view(np.float64)), n, _dp(u.view(np.float64)), _ip(q), len(q))
+
+
+def prob_one(state: np.ndarray, n: int, qubit: int) -> float:
+    return float(lib().or_prob_one(_dp(state.view(np.float64)), n, qubit))
+
+
+def collapse(state: np.ndarray, n: int, qubit: int, outcome: int, p1: float):
+    lib().or_collapse(_dp(state.view(np.float64)), n, qubit, outcome, p1)
+
+
+def marginal_probs(state: np.ndarray, n: int, qubits, literal=False) -> np.ndarray:
+    q = np.ascontiguousarray(qubits, dtype=np.int32)
+    out = np.zeros(1 << len(q), dtype=np.float64)
+    f = lib().or_marginal_probs_literal if literal else lib().or_marginal_probs
+    f(_dp(state.view(np.float64)), n, _ip(q), len(q), _dp(out))
+    return out
+
+
+def sample_counts(probs: np.ndarray, k: int, shots: int, rng: Rng) -> dict:
+    idx = np.zeros(shots, dtype=np.uint32)
+    lib().or_sample_counts(_dp(np.ascontiguousarray(probs)), k, shots, rng._buf,
+                           idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    out: dict[str, int] = {}
+    for v, c in zip(*np.unique(idx, return_counts=True)):
+        out[format(int(v), "b").zfill(k)] = int(c)
+    return out
+
+
+def inner(a: np.ndarray, b: np.ndarray) -> complex:
+    re, im = ctypes.c_double(), ctypes.c_double()
+    lib().or_inner(_dp(a.view(np.float64)), _dp(b.view(np.float64)), a.size,
+                   ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+# -- simulator.rs ------------------------------------------------------------
+
+def num_qubits(circ: dict) -> int:
+    """types.rs:24-30: max(base + size) over qregs, 0 if none."""
+    return max((r["base"] + r["size"] for r in circ.get("qregs", {}).values()), default=0)
+
+
+class OracleResult:
+    def __init__(self, state, n, cbits, p1s):
+        self.statevector = state
+        self.num_qubits = n
+        self.classical_bits = cbits
+        self.p1_record = p1s  # p1 seen by every measurement, in program order
+
+    def probabilities(self, qubits=None) -> dict:
+        qs = list(range(self.num_qubits - 1, -1, -1)) if qubits is None else list(qubits)
+        pr = marginal_probs(self.statevector, self.num_qubits, qs)
+        return {j: float(p) for j, p in enumerate(pr) if p > 0.0}
+
+    def counts(self, shots=1000, qubits=None, seed=None) -> dict:
+        qs = list(range(self.num_qubits - 1, -1, -1)) if qubits is None else list(qubits)
+        pr = marginal_probs(self.statevector, self.num_qubits, qs)
+        return sample_counts(pr, len(qs), shots, Rng(seed))
+
+    def fidelity(self, other: np.ndarray) -> float:
+        if other.size != self.statevector.size:
+            return 0.0
+        z = inner(self.statevector, np.ascontiguousarray(other, dtype=np.complex128))
+        return z.real * z.real + z.imag * z.imag
+
+
+class OracleSimulator:
+    """Restatement of StatevectorSimulator::simulate (simulator.rs:325-379)."""
+
+    def __init__(self, seed: int, parallel: bool = False):
+        self.seed = seed
+        self.parallel = parallel
+
+    def simulate_json(self, text: str, max_instructions: int | None = None) -> OracleResult:
+        circ = json.loads(text)
+        n = num_qubits(circ)
+        state = np.zeros(1 << n, dtype=np.complex128)
+        state[0] = 1.0
+        cbits: dict[int, int] = {}
+        rng = Rng(self.seed)
+        p1s: list[float] = []
+        insts = circ["instructions"]
+        if max_instructions is not None:
+            insts = insts[:max_instructions]
+        for inst in insts:
+            self._run(state, n, inst, cbits, rng, p1s)
+        return OracleResult(state, n, cbits, p1s)
+
+    def simulate(self, circuit: Any) -> OracleResult:
+        return self.simulate_json(circuit.model_dump_json())
+
+    # run_instruction, simulator.rs:386-632
+    def _run(self, state, n, inst, cbits, rng, p1s):
+        kind = inst["kind"]
+        if kind in ("id", "u0", "barrier", "classical"):
+            return
+        if kind == "measure":
+            cbits[inst["cbit"]] = self._measure(state, n, inst["qubit"], rng, p1s)
+            return
+        if kind == "reset":
+            if self._measure(state, n, inst["qubit"], rng, p1s) == 1:
+                self._oq(state, n, gate_matrix("x"), inst["qubit"])
+            return
+        if kind == "conditional":
+            c = inst["condition"]
+            actual = 0
+            for b in range(c["creg_size"]):
+                actual |= cbits.get(c["creg_base"] + b, 0) << b
+            if actual == c["creg_value"]:
+                self._run(state, n, inst["op"], cbits, rng, p1s)
+            return
+        if kind == "gate":
+            name = inst["name"].lower()
+            if name in ("remote_barrier", "remote_cu1"):
+                return
+            if name not in _GENERIC:
+                raise NotImplementedError(
+                    f"Unsupported generic gate: {name!r}. Decompose it before simulating.")
+            self._nq(state, n, gate_matrix(_GENERIC[name]), inst["qubits"])
+            return
+        params = [float(inst[f]) for f in _PARAM_FIELDS.get(kind, ())]
+        qubits = [int(inst[f]) for f in _QUBIT_FIELDS[kind]]
+        u = gate_matrix(kind, params)
+        if len(qubits) == 1:
+            self._oq(state, n, u, qubits[0])
+        else:
+            self._nq(state, n, u, qubits)
+
+    def _oq(self, state, n, u, q):
+        if self.parallel and n >= 12:
+            u = np.ascontiguousarray(u, dtype=np.complex128)
+            lib().or_apply_one_qubit_par(_dp(state.view(np.float64)), n, _dp(u.view(np.float64)), q)
+        else:
+            apply_one_qubit(state, n, u, q)
+
+    def _nq(self, state, n, u, qs):
+        apply_n_qubit(state, n, u, qs, parallel=self.parallel and n >= 12)
+
+    def _measure(self, state, n, q, rng, p1s):
+        d = _dp(state.view(np.float64))
+        if self.parallel and n >= 12:
+            p1 = float(lib().or_prob_one_par(d, n, q))
+        else:
+            p1 = float(lib().or_prob_one(d, n, q))
+        outcome = 1 if rng.gen_f64() < p1 else 0
+        if self.parallel and n >= 12:
+            lib().or_collapse_par(d, n, q, outcome, p1)
+        else:
+            lib().or_collapse(d, n, q, outcome, p1)
+        p1s.append(p1)
+        return outcome
