@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark of the statevector hot path on one MI355X.
+
+Workload: the at-scale plain FP64 random circuit -- 30 qubits, complex128
+(16 GiB state), 40 layers of 61 gates (a random gate from {h,s,t,sx,rx,ry,rz}
+on every qubit, {cz,rzz,cp} on a random perfect matching, 8
+remote_link_psi_plus + 8 nonlocal_cz across the two 15-qubit halves), with
+2 measurements + 2 resets + 1 conditional x after every 10th layer.
+
+One step = reset the resident device state to |0...0> and run the whole
+circuit through the dispatcher (gates, mid-circuit measurements with host
+ChaCha8 draws, resets, conditional); the circuit is decoded once, before
+timing.  value = circuit gates per second over the K timed steps.
+
+roofline: the dominant kernel is the LDS-tiled sweep (k_tile_pass); its
+launches are bracketed by HIP events on the engine's stream.  Algorithmic
+bytes per launch = one read + one write of every amplitude = 2 * 16 * 2^n.
+
+cpu_baseline: the oracle's OpenMP restatement of the reference's Rayon
+kernels (src/engine.rs) on the host cores, on a bounded sample: the circuit's
+gates in program order from |0...0> until ~15 s have elapsed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "30q FP64 gates/s + % HBM peak; adjoint grads/s vs plain param-shift"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(n: int):
+    """HBM bytes per tile-pass launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", f"tile_pass_{n}q_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(circ, n: int, budget_s: float) -> dict:
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    sim = orc.OracleSimulator(seed=42, parallel=True)
+    state = np.zeros(1 << n, dtype=np.complex128)
+    state[0] = 1.0
+    insts = json.loads(circ.model_dump_json())["instructions"]
+    cbits, rng, p1s = {}, orc.Rng(42), []
+    done = 0
+    t0 = time.perf_counter()
+    for inst in insts:
+        if inst["kind"] in ("measure", "reset", "conditional"):
+            break
+        sim._run(state, n, inst, cbits, rng, p1s)
+        done += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "gates/s", "cores": int(orc.lib().or_num_threads()),
+            "kind": "port",
+            "sample": f"first {done} gates of the same {n}-qubit circuit from |0...0> "
+                      f"(OpenMP restatement of src/engine.rs, {dt:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--fusion", type=int, default=2)
+    ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only in this build)", file=sys.stderr)
+        return 2
+
+    import dqsim_bosonic_amd as dq  # noqa: F401  (loads the HIP engine)
+    from dqsim_bosonic_amd import _core
+    from dqsim_bosonic_amd import workloads as W
+
+    n = args.qubits
+    circ = W.random_blocks(n, args.depth, seed=args.seed)
+    gates = W.gate_count(circ)
+    text = circ.model_dump_json()
+    log(f"[bench] workload random_blocks n={n} depth={args.depth} gates={gates}")
+
+    eng = _core.Engine(n, args.fusion, args.tile_bits)
+    eng.load(text)
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        rec = eng.step(args.seed)
+        log(f"[bench] warmup {w}: {time.perf_counter() - t:.3f} s record={rec}")
+
+    eng.set_timing(True)
+    sweeps0, launches0, _ = eng.stats()
+    t0 = time.perf_counter()
+    records = []
+    for k in range(args.steps):
+        records.append(eng.step(args.seed))
+        log(f"[bench] step {k} done")
+    t1 = time.perf_counter()
+    kernel_ms, timed = eng.timing()
+    sweeps1, launches1, _ = eng.stats()
+    eng.set_timing(False)
+    assert all(r == records[0] for r in records), "measurement record changed between steps"
+
+    elapsed = t1 - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = gates * args.steps / elapsed
+    sweeps_per_step = (sweeps1 - sweeps0) / args.steps
+    avg_launch_ms = kernel_ms / max(timed, 1)
+    alg_bytes = 2.0 * 16.0 * float(1 << n)
+    achieved = alg_bytes / (avg_launch_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(n),
+                "kernel": "k_tile_pass<double,16>", "avg_launch_ms": round(avg_launch_ms, 4),
+                "launches_timed": int(timed), "sweeps_per_step": sweeps_per_step,
+                "kernel_time_frac_of_step": round(kernel_ms / 1e3 / elapsed, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(circ, n, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "gates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"random_blocks {n}q complex128 depth {args.depth} "
+                               "(61 gates/layer; measure+reset+conditional every 10 layers)",
+                   "qubits": n, "depth": args.depth, "gates": gates, "seed": args.seed,
+                   "fusion": args.fusion, "measurement_record": records[0]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

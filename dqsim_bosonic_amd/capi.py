@@ -21,7 +21,8 @@ ERR_ARG, ERR_OOM, ERR_DEVICE = -1, -2, -3
 EXPORTS = (
     "dq_state_create", "dq_state_destroy", "dq_state_num_qubits", "dq_state_set", "dq_state_reset",
     "dq_apply_1q", "dq_apply_nq", "dq_prob_one", "dq_collapse", "dq_marginal_probs", "dq_inner",
-    "dq_copy_to_host", "dq_synchronize", "dq_set_options", "dq_get_stats", "dq_last_error",
+    "dq_copy_to_host", "dq_synchronize", "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
+    "dq_last_error",
 )
 
 _lib = None
@@ -53,6 +54,8 @@ def lib() -> ctypes.CDLL:
         L.dq_set_options.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.dq_get_stats.argtypes = [vp, u64p, u64p, u64p]
+        L.dq_set_timing.argtypes = [vp, ctypes.c_int]
+        L.dq_get_timing.argtypes = [vp, _dp, u64p]
         L.dq_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib

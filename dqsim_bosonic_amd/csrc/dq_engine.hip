@@ -360,6 +360,12 @@ struct dq_state {
     double *d_result = nullptr;
     // stats
     uint64_t sweeps = 0, launches = 0, gates = 0;
+    // optional per-launch timing of the sweep kernels (HIP events on `stream`)
+    int timing = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+    std::vector<hipEvent_t> ev_pool;
+    double timed_ms = 0.0;
+    uint64_t timed_launches = 0;
 };
 
 namespace {
@@ -551,8 +557,52 @@ int upload(dq_state *s, const Packed &pk, DevOp **d_ops, void **d_mats, int **d_
     return DQ_OK;
 }
 
+hipEvent_t take_event(dq_state *s) {
+    if (!s->ev_pool.empty()) {
+        hipEvent_t e = s->ev_pool.back();
+        s->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one sweep launch with events when timing is on.
+struct SweepTimer {
+    dq_state *s;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit SweepTimer(dq_state *st) : s(st) {
+        if (!s->timing) return;
+        a = take_event(s);
+        b = take_event(s);
+        if (a) (void)hipEventRecord(a, s->stream);
+    }
+    ~SweepTimer() {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, s->stream);
+        s->ev_pending.emplace_back(a, b);
+    }
+};
+
+int collect_timing(dq_state *s) {
+    if (s->ev_pending.empty()) return DQ_OK;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (auto &pr : s->ev_pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+        s->timed_ms += ms;
+        s->timed_launches++;
+        s->ev_pool.push_back(pr.first);
+        s->ev_pool.push_back(pr.second);
+    }
+    s->ev_pending.clear();
+    return DQ_OK;
+}
+
 template <typename T>
 int launch_tile(dq_state *s, const PassDesc &pd, const DevOp *ops, const void *mats, const int *perms) {
+    SweepTimer timer(s);
     const uint64_t ntiles = uint64_t(1) << pd.n_outer;
     const int nslot = pd.b > 8 ? 1 << (pd.b - 8) : 1;
     const size_t shmem = sizeof(cplx<T>) * size_t(nslot) * kThreads;
@@ -587,6 +637,7 @@ int launch_tile(dq_state *s, const PassDesc &pd, const DevOp *ops, const void *m
 
 template <typename T>
 int launch_direct(dq_state *s, const DevOp *ops, int index, const void *mats, const int *perms) {
+    SweepTimer timer(s);
     const uint64_t work = amp_count(s) / 2;
     hipLaunchKernelGGL(k_direct_op<T>, dim3(grid_for(work)), dim3(kThreads), 0, s->stream,
                        static_cast<cplx<T> *>(s->amps), s->n, ops, index,
@@ -697,6 +748,8 @@ int flush(dq_state *s) {
 }
 
 int enqueue(dq_state *s, HostOp &&op) {
+    if (s->ev_pending.size() > 4096)
+        if (int rc = collect_timing(s)) return rc;
     s->queue.push_back(std::move(op));
     s->gates++;
     if (s->queue.size() >= 4096) return flush(s);
@@ -770,6 +823,11 @@ int dq_state_create(int n_qubits, int dtype, dq_state **out) {
 void dq_state_destroy(dq_state *s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto &pr : s->ev_pending) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     if (s->amps) (void)hipFree(s->amps);
     if (s->d_upload) (void)hipFree(s->d_upload);
     if (s->h_upload) (void)hipHostFree(s->h_upload);
@@ -1046,6 +1104,25 @@ int dq_set_options(dq_state *s, int fuse, int tile_bits) {
     if (fuse < 0 || fuse > 2) return fail(DQ_ERR_ARG, "fusion level must be 0, 1 or 2");
     s->fuse = fuse;
     if (tile_bits) s->tile_bits = tile_bits;
+    return DQ_OK;
+}
+
+int dq_set_timing(dq_state *s, int on) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (int rc = flush(s)) return rc;
+    if (int rc = collect_timing(s)) return rc;
+    s->timing = on ? 1 : 0;
+    s->timed_ms = 0.0;
+    s->timed_launches = 0;
+    return DQ_OK;
+}
+
+int dq_get_timing(dq_state *s, double *sweep_ms, uint64_t *sweep_launches) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (int rc = flush(s)) return rc;
+    if (int rc = collect_timing(s)) return rc;
+    if (sweep_ms) *sweep_ms = s->timed_ms;
+    if (sweep_launches) *sweep_launches = s->timed_launches;
     return DQ_OK;
 }
 

@@ -389,6 +389,51 @@ class StatevectorSimulator {
     int tile_bits_;
 };
 
+// Runs one decoded circuit repeatedly on a resident device state: the same
+// dispatcher as simulate(), without per-call allocation or JSON decoding.
+// Used by the benchmark (one step = reset to |0...0> + the whole circuit).
+class Engine {
+   public:
+    Engine(int n, int fusion, int tile_bits) : n_(n), st_(std::make_shared<DeviceState>(n)) {
+        check(dq_set_options(st_->s, fusion, tile_bits), "options");
+    }
+    void load(const std::string &text) {
+        try {
+            circ_ = Decoder().decode(text);
+        } catch (const JsonError &e) {
+            throw py::value_error(std::string("Circuit JSON parse error: ") + e.what());
+        }
+        if (circ_.num_qubits() != uint64_t(n_)) throw py::value_error("circuit width differs from the engine's");
+    }
+    py::dict step(uint64_t seed) {
+        check(dq_state_reset(st_->s), "reset");
+        Runner run(st_->s, n_, seed, false);
+        for (const Instr &in : circ_.instructions) run.run(in);
+        check(dq_synchronize(st_->s), "synchronize");
+        py::dict d;
+        for (const auto &kv : run.cbits) d[py::int_(kv.first)] = py::int_(kv.second);
+        return d;
+    }
+    void set_timing(bool on) { check(dq_set_timing(st_->s, on ? 1 : 0), "timing"); }
+    py::tuple timing() {
+        double ms = 0;
+        uint64_t cnt = 0;
+        check(dq_get_timing(st_->s, &ms, &cnt), "timing");
+        return py::make_tuple(ms, cnt);
+    }
+    py::tuple stats() {
+        uint64_t a = 0, b = 0, c = 0;
+        check(dq_get_stats(st_->s, &a, &b, &c), "stats");
+        return py::make_tuple(a, b, c);
+    }
+    SimulationResult result() const { return SimulationResult(st_, n_, {}, py::none()); }
+
+   private:
+    int n_;
+    std::shared_ptr<DeviceState> st_;
+    Circuit circ_;
+};
+
 enum class Mode { StateVector, Mps, Stabilizer };
 
 // codecs.rs:19-28
@@ -476,6 +521,15 @@ PYBIND11_MODULE(_core, m) {
         },
         py::arg("circuit"), py::arg("mode") = "state_vector", py::arg("seed") = py::none(),
         py::arg("profile") = false);
+
+    py::class_<Engine>(m, "Engine")
+        .def(py::init<int, int, int>(), py::arg("n"), py::arg("fusion") = 2, py::arg("tile_bits") = 0)
+        .def("load", &Engine::load)
+        .def("step", &Engine::step, py::arg("seed"))
+        .def("set_timing", &Engine::set_timing)
+        .def("timing", &Engine::timing)
+        .def("stats", &Engine::stats)
+        .def("result", &Engine::result);
 
     // Host-logic hooks used by the CPU test-suite (no device needed).
     m.def("_gate_matrix", [](const std::string &name, std::vector<double> params) {

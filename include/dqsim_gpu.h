@@ -16,7 +16,7 @@
  * Execution model: gate applications are queued on the state's stream and
  * may be fused (several gates applied in one HBM sweep, in program order, with
  * each gate's arithmetic unchanged).  Every call that returns data to the host
- * (dq_prob_one, dq_marginal_probs, dq_inner, dq_expect_*, dq_copy_to_host)
+ * (dq_prob_one, dq_marginal_probs, dq_inner, dq_copy_to_host)
  * first flushes the queue and synchronises.  A handle is single-stream and not
  * re-entrant: calls on one handle must not overlap.
  *
@@ -103,6 +103,13 @@ int dq_synchronize(dq_state *s);
  *             arithmetic, last-bit rounding may differ).
  *   tile_bits = 0 keeps the current tile size (6..13 qubits per tile). */
 int dq_set_options(dq_state *s, int fuse, int tile_bits);
+
+/* Sweep-kernel timing (no reference counterpart; measurement only).  With
+ * timing on, every state-sweep kernel launch is bracketed by HIP events on
+ * the state's stream; dq_get_timing flushes, synchronises and returns the
+ * summed kernel time and the number of timed launches since dq_set_timing. */
+int dq_set_timing(dq_state *s, int on);
+int dq_get_timing(dq_state *s, double *sweep_ms, uint64_t *sweep_launches);
 
 /* Number of HBM sweeps and kernel launches issued so far on this handle. */
 int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches,

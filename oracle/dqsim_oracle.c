@@ -85,14 +85,19 @@ void or_rng_from_seed(or_rng *r, const uint8_t seed[32], int rounds) {
 
 /* rand_core 0.6 SeedableRng::seed_from_u64: PCG32 (XSH-RR) fills the seed in
  * 4-byte little-endian chunks, advancing the LCG before each output. */
+/* PCG XSH-RR output permutation of a 64-bit LCG state (O'Neill 2014). */
+uint32_t or_pcg32_output(uint64_t state) {
+    uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
+    uint32_t rot = (uint32_t)(state >> 59);
+    return (xs >> rot) | (xs << ((32 - rot) & 31));
+}
+
 void or_rng_seed_from_u64(or_rng *r, uint64_t state) {
     const uint64_t MUL = 6364136223846793005ull, INC = 11634580027462260723ull;
     uint8_t seed[32];
     for (int c = 0; c < 8; ++c) {
         state = state * MUL + INC;
-        uint32_t xs = (uint32_t)(((state >> 18) ^ state) >> 27);
-        uint32_t rot = (uint32_t)(state >> 59);
-        uint32_t x = (xs >> rot) | (xs << ((32 - rot) & 31));
+        uint32_t x = or_pcg32_output(state);
         seed[4 * c] = (uint8_t)x; seed[4 * c + 1] = (uint8_t)(x >> 8);
         seed[4 * c + 2] = (uint8_t)(x >> 16); seed[4 * c + 3] = (uint8_t)(x >> 24);
     }

@@ -36,6 +36,7 @@ typedef struct {
 
 void or_chacha_block(const uint32_t key[8], uint64_t counter, uint64_t nonce,
                      int rounds, uint32_t out[16]);
+uint32_t or_pcg32_output(uint64_t state);
 void or_rng_from_seed(or_rng *r, const uint8_t seed[32], int rounds);
 void or_rng_seed_from_u64(or_rng *r, uint64_t seed);
 uint32_t or_rng_next_u32(or_rng *r);

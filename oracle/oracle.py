@@ -78,6 +78,9 @@ def lib() -> ctypes.CDLL:
                                       ctypes.c_uint64, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_uint32)]
         L.or_num_threads.restype = ctypes.c_int
+        L.or_pcg32_output.argtypes = [ctypes.c_uint64]
+        L.or_pcg32_output.restype = ctypes.c_uint32
+        L.or_rng_from_seed.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -95,9 +98,12 @@ class Rng:
 
     _SIZE = 8 * 4 + 8 + 64 * 4 + 4 + 4
 
-    def __init__(self, seed: int):
+    def __init__(self, seed: int | None = None, seed_bytes: bytes | None = None, rounds: int = 8):
         self._buf = ctypes.create_string_buffer(self._SIZE + 16)
-        lib().or_rng_seed_from_u64(self._buf, ctypes.c_uint64(seed & (2**64 - 1)))
+        if seed_bytes is not None:
+            lib().or_rng_from_seed(self._buf, seed_bytes, rounds)
+        else:
+            lib().or_rng_seed_from_u64(self._buf, ctypes.c_uint64(seed & (2**64 - 1)))
 
     def next_u64(self) -> int:
         return int(lib().or_rng_next_u64(self._buf))
