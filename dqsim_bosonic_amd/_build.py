@@ -44,9 +44,24 @@ def core_path() -> str:
     return os.path.join(HERE, "_core" + suffix)
 
 
+ENGINE_SOURCES = ("dq_engine.hip", "dq_tile_f64.hip", "dq_tile_f32.hip")
+
+
 def build_engine() -> str:
+    """Compiles the HIP translation units in parallel, then links them."""
+    from concurrent.futures import ThreadPoolExecutor
+
     out = engine_path()
-    _run([HIPCC, *HIP_FLAGS, os.path.join(HERE, "csrc", "dq_engine.hip"), "-o", out])
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    jobs = []
+    for src in ENGINE_SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        jobs.append(([HIPCC, *compile_flags, "-c", os.path.join(HERE, "csrc", src), "-o", obj], obj))
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[o for _, o in jobs], "-o", out])
     return out
 
 

@@ -34,6 +34,9 @@
 
 #include "../../include/dqsim_gpu.h"
 #include "dq_device.hpp"
+#include "dq_tile.hpp"
+#include "dq_tile_launch.hpp"
+#include <type_traits>
 
 using namespace dq;
 
@@ -319,8 +322,9 @@ struct HostOp {
     uint64_t cmask = 0, cval = 0;  // global masks (K_CTRL1)
     int outcome = 0;
     double norm = 1.0;
-    std::vector<double> mat;  // (re, im) pairs: 4 (2x2), 2^k (diag/mono), 4^k (dense)
-    std::vector<int> perm;    // K_MONO
+    std::vector<double> mat;    // (re, im) pairs: 4 (2x2), 2^k (diag/mono), 4^k (dense)
+    std::vector<int> perm;      // K_MONO
+    std::vector<double> dense;  // the full 2^k x 2^k matrix as given (register/LDS paths)
     uint64_t qmask() const {
         uint64_t m = 0;
         for (int j = 0; j < k; ++j) m |= uint64_t(1) << q[j];
@@ -346,6 +350,7 @@ struct dq_state {
     hipStream_t stream = nullptr;
     int fuse = 2;
     int tile_bits = 12;
+    int reg_tiles = 1;  // register-resident tile kernel for tiles of >= 9 qubits
     int low_bits = 5;
     std::vector<HostOp> queue;
     // upload buffers
@@ -395,6 +400,7 @@ bool is_zero(const double *c) { return c[0] == 0.0 && c[1] == 0.0; }
 void classify(HostOp &op, const double *u) {
     const int dim = 1 << op.k;
     auto at = [&](int r, int c) { return u + 2 * (size_t(r) * dim + c); };
+    op.dense.assign(u, u + 2 * size_t(dim) * dim);
     bool diag = true, mono = true;
     std::vector<int> perm(dim, -1), colcount(dim, 0);
     for (int r = 0; r < dim; ++r) {
@@ -419,27 +425,16 @@ void classify(HostOp &op, const double *u) {
         }
         return;
     }
-    if (mono) {
-        op.kind = K_MONO;
-        op.mat.resize(2 * dim);
-        op.perm = perm;
-        for (int r = 0; r < dim; ++r) {
-            op.mat[2 * r] = at(r, perm[r])[0];
-            op.mat[2 * r + 1] = at(r, perm[r])[1];
-        }
-        return;
-    }
     if (op.k == 1) {
         op.kind = K_DENSE1;
         op.mat.assign(u, u + 8);
         return;
     }
-    // controlled 2x2: identity except the bottom-right 2x2 block
+    // controlled 2x2: identity outside the bottom-right 2x2 block
     bool ctrl = true;
     for (int r = 0; r < dim && ctrl; ++r)
         for (int c = 0; c < dim; ++c) {
-            const bool in_block = r >= dim - 2 && c >= dim - 2;
-            if (in_block) continue;
+            if (r >= dim - 2 && c >= dim - 2) continue;
             const double *e = at(r, c);
             const bool want_one = (r == c);
             if (want_one ? !(e[0] == 1.0 && e[1] == 0.0) : !is_zero(e)) { ctrl = false; break; }
@@ -456,6 +451,16 @@ void classify(HostOp &op, const double *u) {
         for (int j = 0; j < op.k - 1; ++j) {
             op.cmask |= uint64_t(1) << op.q[j];
             op.cval |= uint64_t(1) << op.q[j];
+        }
+        return;
+    }
+    if (mono) {
+        op.kind = K_MONO;
+        op.mat.resize(2 * dim);
+        op.perm = perm;
+        for (int r = 0; r < dim; ++r) {
+            op.mat[2 * r] = at(r, perm[r])[0];
+            op.mat[2 * r + 1] = at(r, perm[r])[1];
         }
         return;
     }
@@ -649,10 +654,18 @@ int launch_direct(dq_state *s, const DevOp *ops, int index, const void *mats, co
 }
 
 // Greedy pass builder (see the file header).
+// Tile size of the passes: the configured size, or the whole state if it is
+// smaller; register-resident tiles exist for 9, 12 and 13 qubits.
+int effective_tile_bits(const dq_state *s) {
+    const int b = std::min(s->tile_bits, s->n);
+    if (s->fuse && s->reg_tiles && b >= 9 && !dq_tile_bits_supported(b)) return b < 12 ? 9 : 12;
+    return b;
+}
+
 std::vector<Pass> schedule(const dq_state *s) {
     std::vector<Pass> passes;
     const int n = s->n;
-    const int b = std::min(s->tile_bits, n);
+    const int b = effective_tile_bits(s);
     // leave room for a 5-qubit gate on qubits outside the low block
     const int low = b < n ? std::min(s->low_bits, b - 5) : b;
     const uint64_t low_mask = (low >= 64) ? ~uint64_t(0) : ((uint64_t(1) << low) - 1);
@@ -688,6 +701,261 @@ std::vector<Pass> schedule(const dq_state *s) {
     return passes;
 }
 
+// ---- register-resident tile passes (k_tile_reg) ----
+
+constexpr int kLowQubits = 5;  // qubits 0..4 stay thread bits at load/store (coalescing)
+
+int pack_rb(unsigned mask) {
+    int packed = 0, m = 0;
+    for (int bpos = 0; bpos < 16 && m < 4; ++bpos)
+        if ((mask >> bpos) & 1) packed |= bpos << (4 * m++);
+    return packed;
+}
+
+unsigned unpack_rb(int packed) {
+    unsigned m = 0;
+    for (int j = 0; j < 4; ++j) m |= 1u << ((packed >> (4 * j)) & 15);
+    return m;
+}
+
+int rb_slot(int packed, int tile_bit) {
+    for (int j = 0; j < 4; ++j)
+        if (((packed >> (4 * j)) & 15) == tile_bit) return j;
+    return -1;
+}
+
+unsigned long long entry_modes(const std::vector<double> &m) {
+    unsigned long long modes = 0;
+    for (size_t e = 0; e < m.size() / 2 && e < 21; ++e) {
+        const double re = m[2 * e], im = m[2 * e + 1];
+        int md;
+        if (re == 0.0 && im == 0.0) md = M_ZERO;
+        else if (re == 1.0 && im == 0.0) md = M_ONE;
+        else if (re == -1.0 && im == 0.0) md = M_MONE;
+        else if (im == 0.0) md = M_REAL;
+        else if (re == 0.0) md = M_IMAG;
+        else md = M_GEN;
+        modes |= (unsigned long long)md << (3 * e);
+    }
+    return modes;
+}
+
+std::vector<double> diag_to_dense(const std::vector<double> &d) {
+    const size_t dim = d.size() / 2;
+    std::vector<double> m(2 * dim * dim, 0.0);
+    for (size_t r = 0; r < dim; ++r) {
+        m[2 * (r * dim + r)] = d[2 * r];
+        m[2 * (r * dim + r) + 1] = d[2 * r + 1];
+    }
+    return m;
+}
+
+struct RegPlan {
+    TilePass pd;
+    int b;
+    std::vector<TileOp> ops;
+};
+
+// Plans the layout of every op of one pass and encodes the ops.
+void plan_pass(const dq_state *s, const Pass &p, std::vector<double> &mats, RegPlan &out) {
+    const int n = s->n;
+    TilePass &pd = out.pd;
+    std::memset(&pd, 0, sizeof pd);
+    int pos[64];
+    int tb = 0, ob = 0;
+    for (int q = 0; q < n; ++q) {
+        if ((p.S >> q) & 1) {
+            pd.tq[tb] = q;
+            pos[q] = tb++;
+        } else {
+            pd.oq[ob++] = q;
+            pos[q] = -1;
+        }
+    }
+    const int b = tb;
+    out.b = b;
+    pd.n_outer = ob;
+    const unsigned lowmask = (1u << kLowQubits) - 1;
+    const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - 4)) - 1);
+    const size_t nops = p.ops.size();
+    std::vector<unsigned> hard(nops, 0), soft(nops, 0);
+    for (size_t i = 0; i < nops; ++i) {
+        const HostOp &h = s->queue[p.ops[i]];
+        unsigned qm = 0;
+        for (int j = 0; j < h.k; ++j) qm |= 1u << pos[h.q[j]];
+        switch (h.kind) {
+        case K_DENSE1:
+        case K_MONO:
+        case K_DENSEK:
+            if (h.k <= 2) hard[i] = qm;
+            break;
+        case K_CTRL1:
+            hard[i] = 1u << pos[h.q[h.k - 1]];
+            break;
+        case K_DIAG:
+            if (h.k <= 2) soft[i] = qm;
+            break;
+        default:
+            break;
+        }
+    }
+    auto build = [&](size_t i, unsigned curmask) {
+        unsigned S = hard[i];
+        for (size_t j = i + 1; j < nops && j < i + 64; ++j) {
+            const unsigned need = hard[j] ? hard[j] : soft[j];
+            if (!need || (need & ~S) == 0) continue;
+            if (__builtin_popcount(S | need) <= 4) S |= need;
+            else if (hard[j]) break;
+        }
+        // fill to 4 bits: keep current register bits, then high tile bits, then the rest
+        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos)
+            if (((curmask >> bpos) & 1) && !((lowmask >> bpos) & 1)) S |= 1u << bpos;
+        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos)
+            if (!((lowmask >> bpos) & 1)) S |= 1u << bpos;
+        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos) S |= 1u << bpos;
+        return S;
+    };
+    std::vector<int> lay(nops, -1), relay(nops, 0);
+    int cur = -1;
+    for (size_t i = 0; i < nops; ++i) {
+        if (hard[i] && (cur < 0 || (hard[i] & ~unpack_rb(cur)))) {
+            cur = pack_rb(build(i, cur < 0 ? 0u : unpack_rb(cur)));
+            relay[i] = 1;
+        }
+        lay[i] = cur;
+    }
+    // load layout: the first planned layout if it keeps the low qubits as thread bits
+    size_t first = nops;
+    for (size_t i = 0; i < nops; ++i)
+        if (relay[i]) { first = i; break; }
+    if (first < nops && !(unpack_rb(lay[first]) & lowmask)) {
+        pd.load_rb = lay[first];
+        relay[first] = 0;
+    } else {
+        pd.load_rb = pack_rb(dflt);
+    }
+    for (size_t i = 0; i < first && i < nops; ++i) lay[i] = pd.load_rb;
+    const int last = nops ? lay[nops - 1] : pd.load_rb;
+    pd.store_rb = (unpack_rb(last) & lowmask) ? pack_rb(dflt) : last;
+
+    for (size_t i = 0; i < nops; ++i) {
+        const HostOp &h = s->queue[p.ops[i]];
+        TileOp t;
+        std::memset(&t, 0, sizeof t);
+        t.k = h.k;
+        t.relayout = relay[i];
+        t.rb = lay[i];
+        for (int j = 0; j < h.k; ++j) t.bit[j] = pos[h.q[j]];
+        for (int j = 0; j < h.k; ++j) t.sbit[j] = t.bit[j];
+        std::sort(t.sbit, t.sbit + h.k);
+        const unsigned regmask = unpack_rb(lay[i]);
+        unsigned qm = 0;
+        for (int j = 0; j < h.k; ++j) qm |= 1u << t.bit[j];
+        const std::vector<double> *m = &h.mat;
+        std::vector<double> tmp;
+        switch (h.kind) {
+        case K_DENSE1:
+        case K_MONO:
+        case K_DENSEK:
+            if (h.k == 1) {
+                t.kind = T_DENSE1;
+                t.p = rb_slot(lay[i], t.bit[0]);
+                m = &h.dense;
+            } else if (h.k == 2) {
+                t.kind = T_DENSE2;
+                t.p = rb_slot(lay[i], t.bit[0]);
+                t.q = rb_slot(lay[i], t.bit[1]);
+                m = &h.dense;
+            } else {
+                t.kind = T_LDSOP;
+                m = &h.dense;
+            }
+            break;
+        case K_CTRL1:
+            t.kind = T_CTRL1;
+            t.p = rb_slot(lay[i], t.bit[h.k - 1]);
+            for (int j = 0; j < h.k - 1; ++j) {
+                t.cmask |= 1u << t.bit[j];
+                if (h.cval & (uint64_t(1) << h.q[j])) t.cval |= 1u << t.bit[j];
+            }
+            break;
+        case K_DIAG:
+            if (h.k <= 2 && (qm & ~regmask) == 0) {
+                tmp = diag_to_dense(h.mat);
+                m = &tmp;
+                t.kind = h.k == 1 ? T_DENSE1 : T_DENSE2;
+                t.p = rb_slot(lay[i], t.bit[0]);
+                if (h.k == 2) t.q = rb_slot(lay[i], t.bit[1]);
+            } else {
+                t.kind = T_DIAG;
+            }
+            break;
+        case K_COLLAPSE:
+            t.kind = T_COLLAPSE;
+            t.outcome = h.outcome;
+            t.norm = h.norm;
+            break;
+        default:
+            break;
+        }
+        t.mat = int(mats.size() / 2);
+        mats.insert(mats.end(), m->begin(), m->end());
+        t.modes = entry_modes(*m);
+        out.ops.push_back(t);
+    }
+}
+
+template <typename T>
+int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *mats) {
+    SweepTimer timer(s);
+    const uint64_t ntiles = uint64_t(1) << rp.pd.n_outer;
+    const uint64_t grid = std::min<uint64_t>(ntiles, 16384);
+    hipError_t e = std::is_same<T, double>::value
+                       ? dq_launch_tile_reg_f64(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles)
+                       : dq_launch_tile_reg_f32(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles);
+    if (e == hipErrorInvalidValue) return fail(DQ_ERR_ARG, "unsupported tile size " + std::to_string(rp.b));
+    HIP_TRY(e);
+    s->launches++;
+    s->sweeps++;
+    return DQ_OK;
+}
+
+template <typename T>
+int flush_reg(dq_state *s, const std::vector<Pass> &passes) {
+    std::vector<RegPlan> plans(passes.size());
+    std::vector<double> mats;
+    size_t total_ops = 0;
+    for (size_t i = 0; i < passes.size(); ++i) {
+        plan_pass(s, passes[i], mats, plans[i]);
+        plans[i].pd.op_begin = int(total_ops);
+        plans[i].pd.op_count = int(plans[i].ops.size());
+        total_ops += plans[i].ops.size();
+    }
+    const size_t real_b = sizeof(T);
+    const size_t ops_b = align_up(std::max<size_t>(total_ops, 1) * sizeof(TileOp), 256);
+    const size_t mats_b = align_up(std::max<size_t>(mats.size(), 2) * real_b, 256);
+    const size_t total = ops_b + mats_b;
+    HIP_TRY(hipEventSynchronize(s->upload_done));
+    if (int rc = ensure_host_buffer(s, total)) return rc;
+    if (int rc = ensure_device_buffer(s, total)) return rc;
+    unsigned char *h = s->h_upload;
+    size_t off = 0;
+    for (const RegPlan &rp : plans) {
+        std::memcpy(h + off, rp.ops.data(), rp.ops.size() * sizeof(TileOp));
+        off += rp.ops.size() * sizeof(TileOp);
+    }
+    T *hm = reinterpret_cast<T *>(h + ops_b);
+    for (size_t i = 0; i < mats.size(); ++i) hm[i] = T(mats[i]);
+    HIP_TRY(hipMemcpyAsync(s->d_upload, h, total, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipEventRecord(s->upload_done, s->stream));
+    const TileOp *d_ops = reinterpret_cast<const TileOp *>(s->d_upload);
+    const void *d_mats = static_cast<unsigned char *>(s->d_upload) + ops_b;
+    for (const RegPlan &rp : plans)
+        if (int rc = launch_reg<T>(s, rp, d_ops, d_mats)) return rc;
+    s->queue.clear();
+    return DQ_OK;
+}
+
 template <typename T>
 int flush_t(dq_state *s) {
     if (s->queue.empty()) return DQ_OK;
@@ -695,6 +963,9 @@ int flush_t(dq_state *s) {
     Packed pk;
     std::vector<PassDesc> descs;
     int rc = DQ_OK;
+    if (s->fuse && s->reg_tiles && effective_tile_bits(s) >= 9) {
+        return flush_reg<T>(s, schedule(s));
+    }
     if (s->fuse && n >= 1) {
         std::vector<Pass> passes = schedule(s);
         for (const Pass &p : passes) {
@@ -904,6 +1175,7 @@ int dq_apply_1q(dq_state *s, const double u[8], int target, const int *ctrl_wire
     }
     op.q[n_ctrl] = target;
     op.mat.assign(u, u + 8);
+    op.dense.assign(u, u + 8);
     return enqueue(s, std::move(op));
 }
 

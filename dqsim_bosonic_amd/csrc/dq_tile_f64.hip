@@ -1,0 +1,33 @@
+// dq_tile_f64.hip -- complex128 instantiations of the register-resident tile kernel.
+#include "dq_tile_launch.hpp"
+
+using namespace dq;
+
+namespace {
+
+template <int B>
+hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass &pd, const TileOp *ops,
+                  const void *mats, uint64_t ntiles) {
+    using T = double;
+    const size_t shmem = sizeof(cplx<T>) << B;
+    if (shmem > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg<T, B>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_tile_reg<T, B>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
+                       static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t dq_launch_tile_reg_f64(int b, unsigned grid, hipStream_t stream, void *amps, const TilePass &pd,
+                                   const TileOp *ops, const void *mats, uint64_t ntiles) {
+    switch (b) {
+    case 9: return launch<9>(grid, stream, amps, pd, ops, mats, ntiles);
+    case 12: return launch<12>(grid, stream, amps, pd, ops, mats, ntiles);
+    case 13: return launch<13>(grid, stream, amps, pd, ops, mats, ntiles);
+    default: return hipErrorInvalidValue;
+    }
+}

@@ -1,0 +1,16 @@
+// dq_tile_launch.hpp -- host launchers of the register-resident tile kernel.
+// Each precision is compiled in its own translation unit (dq_tile_f64.hip,
+// dq_tile_f32.hip); the engine calls these host functions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dq_tile.hpp"
+
+// Tile sizes with a compiled kernel; any other returns hipErrorInvalidValue.
+inline bool dq_tile_bits_supported(int b) { return b == 9 || b == 12 || b == 13; }
+
+hipError_t dq_launch_tile_reg_f64(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
+                                  const dq::TileOp *ops, const void *mats, uint64_t ntiles);
+hipError_t dq_launch_tile_reg_f32(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
+                                  const dq::TileOp *ops, const void *mats, uint64_t ntiles);
