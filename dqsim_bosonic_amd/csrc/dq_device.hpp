@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dq_types.hpp"
+
 namespace dq {
 
 template <typename T>
@@ -27,31 +29,6 @@ template <typename T>
 __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
     return {a.re + b.re, a.im + b.im};
 }
-
-enum OpKind : int {
-    K_DENSE1 = 0,   // 2x2 on bit[0]
-    K_DIAG = 1,     // diagonal 2^k, entries at mat
-    K_MONO = 2,     // monomial: out[r] = ph[r] * in[perm[r]], ph at mat, perm at aux
-    K_CTRL1 = 3,    // 2x2 on bit[k-1] where (i & cmask) == cval
-    K_DENSEK = 4,   // dense 2^k x 2^k, acc = 0; acc += U[r][c] * v[c]
-    K_COLLAPSE = 5  // keep bit[0] == outcome (divide by norm), zero the rest
-};
-
-// One queued operation, positions relative to the index space it runs on
-// (global qubits for direct launches, tile-local bits inside a tile pass).
-struct DevOp {
-    int kind;
-    int k;
-    int mat;          // offset (complex entries) into the matrix pool
-    int aux;          // offset (ints) into the perm pool (K_MONO)
-    int bit[5];       // bit[0] = MSB of the gate index
-    int sbit[5];      // the same positions, ascending
-    int outcome;
-    int pad;
-    uint64_t cmask;   // K_CTRL1
-    uint64_t cval;
-    double norm;      // K_COLLAPSE
-};
 
 // Index of the g-th group base: insert a zero at each ascending position.
 template <typename I>

@@ -36,6 +36,7 @@
 #include "dq_device.hpp"
 #include "dq_tile.hpp"
 #include "dq_tile_launch.hpp"
+#include "dq_plan.hpp"
 #include <type_traits>
 
 using namespace dq;
@@ -315,30 +316,6 @@ __global__ __launch_bounds__(kThreads) void k_marginal_tree(const cplx<T> *__res
 // host side
 // ------------------------------------------------------------------------
 
-struct HostOp {
-    int kind = K_DENSE1;
-    int k = 1;
-    int q[5] = {0, 0, 0, 0, 0};  // global qubits, q[0] = MSB of the gate index
-    uint64_t cmask = 0, cval = 0;  // global masks (K_CTRL1)
-    int outcome = 0;
-    double norm = 1.0;
-    std::vector<double> mat;    // (re, im) pairs: 4 (2x2), 2^k (diag/mono), 4^k (dense)
-    std::vector<int> perm;      // K_MONO
-    std::vector<double> dense;  // the full 2^k x 2^k matrix as given (register/LDS paths)
-    uint64_t qmask() const {
-        uint64_t m = 0;
-        for (int j = 0; j < k; ++j) m |= uint64_t(1) << q[j];
-        return m;
-    }
-};
-
-struct Pass {
-    uint64_t S = 0;  // tile qubits (mask); 0 = direct launch of a single op
-    std::vector<int> ops;
-};
-
-int popcount64(uint64_t x) { return __builtin_popcountll(x); }
-
 }  // namespace
 
 struct dq_state {
@@ -377,6 +354,16 @@ namespace {
 
 uint64_t amp_count(const dq_state *s) { return uint64_t(1) << s->n; }
 
+PlanConfig plan_config(const dq_state *s) {
+    PlanConfig c;
+    c.n = s->n;
+    c.fuse = s->fuse;
+    c.tile_bits = s->tile_bits;
+    c.low_bits = s->low_bits;
+    c.reg_tiles = s->reg_tiles;
+    return c;
+}
+
 int grid_for(uint64_t work) {
     uint64_t g = (work + kThreads - 1) / kThreads;
     if (g > 65536) g = 65536;
@@ -389,83 +376,6 @@ int check_qubit(const dq_state *s, int q) {
                                                         " out of range for " + std::to_string(s->n) +
                                                         "-qubit state");
     return DQ_OK;
-}
-
-bool is_zero(const double *c) { return c[0] == 0.0 && c[1] == 0.0; }
-
-// Classifies a dense 2^k matrix into the cheapest op kind that reproduces the
-// dense formula exactly (see dq_device.hpp): diagonal (every off-diagonal
-// entry is exactly zero), monomial (one nonzero per row, a permutation),
-// controlled-2x2 (identity outside the bottom-right 2x2 block), else dense.
-void classify(HostOp &op, const double *u) {
-    const int dim = 1 << op.k;
-    auto at = [&](int r, int c) { return u + 2 * (size_t(r) * dim + c); };
-    op.dense.assign(u, u + 2 * size_t(dim) * dim);
-    bool diag = true, mono = true;
-    std::vector<int> perm(dim, -1), colcount(dim, 0);
-    for (int r = 0; r < dim; ++r) {
-        int nz = 0;
-        for (int c = 0; c < dim; ++c) {
-            if (is_zero(at(r, c))) continue;
-            ++nz;
-            perm[r] = c;
-            if (c != r) diag = false;
-        }
-        if (nz != 1) mono = false;
-        else colcount[perm[r]]++;
-    }
-    for (int c = 0; c < dim && mono; ++c)
-        if (colcount[c] != 1) mono = false;
-    if (diag) {
-        op.kind = K_DIAG;
-        op.mat.resize(2 * dim);
-        for (int r = 0; r < dim; ++r) {
-            op.mat[2 * r] = at(r, r)[0];
-            op.mat[2 * r + 1] = at(r, r)[1];
-        }
-        return;
-    }
-    if (op.k == 1) {
-        op.kind = K_DENSE1;
-        op.mat.assign(u, u + 8);
-        return;
-    }
-    // controlled 2x2: identity outside the bottom-right 2x2 block
-    bool ctrl = true;
-    for (int r = 0; r < dim && ctrl; ++r)
-        for (int c = 0; c < dim; ++c) {
-            if (r >= dim - 2 && c >= dim - 2) continue;
-            const double *e = at(r, c);
-            const bool want_one = (r == c);
-            if (want_one ? !(e[0] == 1.0 && e[1] == 0.0) : !is_zero(e)) { ctrl = false; break; }
-        }
-    if (ctrl) {
-        op.kind = K_CTRL1;
-        op.mat.resize(8);
-        for (int r = 0; r < 2; ++r)
-            for (int c = 0; c < 2; ++c) {
-                op.mat[2 * (2 * r + c)] = at(dim - 2 + r, dim - 2 + c)[0];
-                op.mat[2 * (2 * r + c) + 1] = at(dim - 2 + r, dim - 2 + c)[1];
-            }
-        op.cmask = op.cval = 0;
-        for (int j = 0; j < op.k - 1; ++j) {
-            op.cmask |= uint64_t(1) << op.q[j];
-            op.cval |= uint64_t(1) << op.q[j];
-        }
-        return;
-    }
-    if (mono) {
-        op.kind = K_MONO;
-        op.mat.resize(2 * dim);
-        op.perm = perm;
-        for (int r = 0; r < dim; ++r) {
-            op.mat[2 * r] = at(r, perm[r])[0];
-            op.mat[2 * r + 1] = at(r, perm[r])[1];
-        }
-        return;
-    }
-    op.kind = K_DENSEK;
-    op.mat.assign(u, u + 2 * size_t(dim) * dim);
 }
 
 // ---- upload + launch ----
@@ -654,257 +564,6 @@ int launch_direct(dq_state *s, const DevOp *ops, int index, const void *mats, co
 }
 
 // Greedy pass builder (see the file header).
-// Tile size of the passes: the configured size, or the whole state if it is
-// smaller; register-resident tiles exist for 9, 12 and 13 qubits.
-int effective_tile_bits(const dq_state *s) {
-    const int b = std::min(s->tile_bits, s->n);
-    if (s->fuse && s->reg_tiles && b >= 9 && !dq_tile_bits_supported(b)) return b < 12 ? 9 : 12;
-    return b;
-}
-
-std::vector<Pass> schedule(const dq_state *s) {
-    std::vector<Pass> passes;
-    const int n = s->n;
-    const int b = effective_tile_bits(s);
-    // leave room for a 5-qubit gate on qubits outside the low block
-    const int low = b < n ? std::min(s->low_bits, b - 5) : b;
-    const uint64_t low_mask = (low >= 64) ? ~uint64_t(0) : ((uint64_t(1) << low) - 1);
-    std::vector<int> remaining(s->queue.size());
-    for (size_t i = 0; i < remaining.size(); ++i) remaining[i] = int(i);
-    while (!remaining.empty()) {
-        Pass p;
-        p.S = low_mask;
-        uint64_t blocked = 0;
-        std::vector<int> rest;
-        for (int idx : remaining) {
-            const uint64_t qm = s->queue[idx].qmask();
-            if (qm & blocked) {
-                blocked |= qm;
-                rest.push_back(idx);
-                continue;
-            }
-            if (popcount64(p.S | qm) <= b) {
-                p.S |= qm;
-                p.ops.push_back(idx);
-            } else {
-                blocked |= qm;
-                rest.push_back(idx);
-                // program-order mode: the pass is a prefix of the queue
-                if (s->fuse == 1) blocked = ~uint64_t(0);
-            }
-        }
-        // fill the tile up to b qubits with the lowest unused qubits
-        for (int q = 0; q < n && popcount64(p.S) < b; ++q) p.S |= uint64_t(1) << q;
-        passes.push_back(std::move(p));
-        remaining.swap(rest);
-    }
-    return passes;
-}
-
-// ---- register-resident tile passes (k_tile_reg) ----
-
-constexpr int kLowQubits = 5;  // qubits 0..4 stay thread bits at load/store (coalescing)
-
-int pack_rb(unsigned mask) {
-    int packed = 0, m = 0;
-    for (int bpos = 0; bpos < 16 && m < 4; ++bpos)
-        if ((mask >> bpos) & 1) packed |= bpos << (4 * m++);
-    return packed;
-}
-
-unsigned unpack_rb(int packed) {
-    unsigned m = 0;
-    for (int j = 0; j < 4; ++j) m |= 1u << ((packed >> (4 * j)) & 15);
-    return m;
-}
-
-int rb_slot(int packed, int tile_bit) {
-    for (int j = 0; j < 4; ++j)
-        if (((packed >> (4 * j)) & 15) == tile_bit) return j;
-    return -1;
-}
-
-unsigned long long entry_modes(const std::vector<double> &m) {
-    unsigned long long modes = 0;
-    for (size_t e = 0; e < m.size() / 2 && e < 21; ++e) {
-        const double re = m[2 * e], im = m[2 * e + 1];
-        int md;
-        if (re == 0.0 && im == 0.0) md = M_ZERO;
-        else if (re == 1.0 && im == 0.0) md = M_ONE;
-        else if (re == -1.0 && im == 0.0) md = M_MONE;
-        else if (im == 0.0) md = M_REAL;
-        else if (re == 0.0) md = M_IMAG;
-        else md = M_GEN;
-        modes |= (unsigned long long)md << (3 * e);
-    }
-    return modes;
-}
-
-std::vector<double> diag_to_dense(const std::vector<double> &d) {
-    const size_t dim = d.size() / 2;
-    std::vector<double> m(2 * dim * dim, 0.0);
-    for (size_t r = 0; r < dim; ++r) {
-        m[2 * (r * dim + r)] = d[2 * r];
-        m[2 * (r * dim + r) + 1] = d[2 * r + 1];
-    }
-    return m;
-}
-
-struct RegPlan {
-    TilePass pd;
-    int b;
-    std::vector<TileOp> ops;
-};
-
-// Plans the layout of every op of one pass and encodes the ops.
-void plan_pass(const dq_state *s, const Pass &p, std::vector<double> &mats, RegPlan &out) {
-    const int n = s->n;
-    TilePass &pd = out.pd;
-    std::memset(&pd, 0, sizeof pd);
-    int pos[64];
-    int tb = 0, ob = 0;
-    for (int q = 0; q < n; ++q) {
-        if ((p.S >> q) & 1) {
-            pd.tq[tb] = q;
-            pos[q] = tb++;
-        } else {
-            pd.oq[ob++] = q;
-            pos[q] = -1;
-        }
-    }
-    const int b = tb;
-    out.b = b;
-    pd.n_outer = ob;
-    const unsigned lowmask = (1u << kLowQubits) - 1;
-    const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - 4)) - 1);
-    const size_t nops = p.ops.size();
-    std::vector<unsigned> hard(nops, 0), soft(nops, 0);
-    for (size_t i = 0; i < nops; ++i) {
-        const HostOp &h = s->queue[p.ops[i]];
-        unsigned qm = 0;
-        for (int j = 0; j < h.k; ++j) qm |= 1u << pos[h.q[j]];
-        switch (h.kind) {
-        case K_DENSE1:
-        case K_MONO:
-        case K_DENSEK:
-            if (h.k <= 2) hard[i] = qm;
-            break;
-        case K_CTRL1:
-            hard[i] = 1u << pos[h.q[h.k - 1]];
-            break;
-        case K_DIAG:
-            if (h.k <= 2) soft[i] = qm;
-            break;
-        default:
-            break;
-        }
-    }
-    auto build = [&](size_t i, unsigned curmask) {
-        unsigned S = hard[i];
-        for (size_t j = i + 1; j < nops && j < i + 64; ++j) {
-            const unsigned need = hard[j] ? hard[j] : soft[j];
-            if (!need || (need & ~S) == 0) continue;
-            if (__builtin_popcount(S | need) <= 4) S |= need;
-            else if (hard[j]) break;
-        }
-        // fill to 4 bits: keep current register bits, then high tile bits, then the rest
-        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos)
-            if (((curmask >> bpos) & 1) && !((lowmask >> bpos) & 1)) S |= 1u << bpos;
-        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos)
-            if (!((lowmask >> bpos) & 1)) S |= 1u << bpos;
-        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos) S |= 1u << bpos;
-        return S;
-    };
-    std::vector<int> lay(nops, -1), relay(nops, 0);
-    int cur = -1;
-    for (size_t i = 0; i < nops; ++i) {
-        if (hard[i] && (cur < 0 || (hard[i] & ~unpack_rb(cur)))) {
-            cur = pack_rb(build(i, cur < 0 ? 0u : unpack_rb(cur)));
-            relay[i] = 1;
-        }
-        lay[i] = cur;
-    }
-    // load layout: the first planned layout if it keeps the low qubits as thread bits
-    size_t first = nops;
-    for (size_t i = 0; i < nops; ++i)
-        if (relay[i]) { first = i; break; }
-    if (first < nops && !(unpack_rb(lay[first]) & lowmask)) {
-        pd.load_rb = lay[first];
-        relay[first] = 0;
-    } else {
-        pd.load_rb = pack_rb(dflt);
-    }
-    for (size_t i = 0; i < first && i < nops; ++i) lay[i] = pd.load_rb;
-    const int last = nops ? lay[nops - 1] : pd.load_rb;
-    pd.store_rb = (unpack_rb(last) & lowmask) ? pack_rb(dflt) : last;
-
-    for (size_t i = 0; i < nops; ++i) {
-        const HostOp &h = s->queue[p.ops[i]];
-        TileOp t;
-        std::memset(&t, 0, sizeof t);
-        t.k = h.k;
-        t.relayout = relay[i];
-        t.rb = lay[i];
-        for (int j = 0; j < h.k; ++j) t.bit[j] = pos[h.q[j]];
-        for (int j = 0; j < h.k; ++j) t.sbit[j] = t.bit[j];
-        std::sort(t.sbit, t.sbit + h.k);
-        const unsigned regmask = unpack_rb(lay[i]);
-        unsigned qm = 0;
-        for (int j = 0; j < h.k; ++j) qm |= 1u << t.bit[j];
-        const std::vector<double> *m = &h.mat;
-        std::vector<double> tmp;
-        switch (h.kind) {
-        case K_DENSE1:
-        case K_MONO:
-        case K_DENSEK:
-            if (h.k == 1) {
-                t.kind = T_DENSE1;
-                t.p = rb_slot(lay[i], t.bit[0]);
-                m = &h.dense;
-            } else if (h.k == 2) {
-                t.kind = T_DENSE2;
-                t.p = rb_slot(lay[i], t.bit[0]);
-                t.q = rb_slot(lay[i], t.bit[1]);
-                m = &h.dense;
-            } else {
-                t.kind = T_LDSOP;
-                m = &h.dense;
-            }
-            break;
-        case K_CTRL1:
-            t.kind = T_CTRL1;
-            t.p = rb_slot(lay[i], t.bit[h.k - 1]);
-            for (int j = 0; j < h.k - 1; ++j) {
-                t.cmask |= 1u << t.bit[j];
-                if (h.cval & (uint64_t(1) << h.q[j])) t.cval |= 1u << t.bit[j];
-            }
-            break;
-        case K_DIAG:
-            if (h.k <= 2 && (qm & ~regmask) == 0) {
-                tmp = diag_to_dense(h.mat);
-                m = &tmp;
-                t.kind = h.k == 1 ? T_DENSE1 : T_DENSE2;
-                t.p = rb_slot(lay[i], t.bit[0]);
-                if (h.k == 2) t.q = rb_slot(lay[i], t.bit[1]);
-            } else {
-                t.kind = T_DIAG;
-            }
-            break;
-        case K_COLLAPSE:
-            t.kind = T_COLLAPSE;
-            t.outcome = h.outcome;
-            t.norm = h.norm;
-            break;
-        default:
-            break;
-        }
-        t.mat = int(mats.size() / 2);
-        mats.insert(mats.end(), m->begin(), m->end());
-        t.modes = entry_modes(*m);
-        out.ops.push_back(t);
-    }
-}
-
 template <typename T>
 int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *mats) {
     SweepTimer timer(s);
@@ -922,11 +581,12 @@ int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *ma
 
 template <typename T>
 int flush_reg(dq_state *s, const std::vector<Pass> &passes) {
+    const PlanConfig cfg = plan_config(s);
     std::vector<RegPlan> plans(passes.size());
     std::vector<double> mats;
     size_t total_ops = 0;
     for (size_t i = 0; i < passes.size(); ++i) {
-        plan_pass(s, passes[i], mats, plans[i]);
+        plan_pass(&cfg, s->queue, passes[i], mats, plans[i]);
         plans[i].pd.op_begin = int(total_ops);
         plans[i].pd.op_count = int(plans[i].ops.size());
         total_ops += plans[i].ops.size();
@@ -963,11 +623,12 @@ int flush_t(dq_state *s) {
     Packed pk;
     std::vector<PassDesc> descs;
     int rc = DQ_OK;
-    if (s->fuse && s->reg_tiles && effective_tile_bits(s) >= 9) {
-        return flush_reg<T>(s, schedule(s));
+    const PlanConfig cfg = plan_config(s);
+    if (s->fuse && s->reg_tiles && effective_tile_bits(&cfg) >= 9) {
+        return flush_reg<T>(s, schedule(&cfg, s->queue));
     }
     if (s->fuse && n >= 1) {
-        std::vector<Pass> passes = schedule(s);
+        std::vector<Pass> passes = schedule(&cfg, s->queue);
         for (const Pass &p : passes) {
             PassDesc pd;
             std::memset(&pd, 0, sizeof pd);

@@ -10,51 +10,15 @@
 // layout every op runs in, so most gates cost no LDS traffic and no barrier.
 //
 // Arithmetic: each output is the reference's row sum (src/engine.rs:58-59,
-// 146-151) in the same column order.  Products with an exactly zero entry are
-// skipped and products with entries 1, -1, real or imaginary use only the
-// nonzero parts; the skipped terms are +-0 and do not change any sum, so the
-// values equal the reference's up to the sign of a zero.
+// 146-151) in the same column order.  The host picks a variant per op:
+// general, real (only the real parts of the entries multiply), diagonal or
+// anti-diagonal (products of zero entries dropped).  Dropped terms are +-0
+// and do not change any sum, so the values equal the reference's up to the
+// sign of a zero.
 #pragma once
 #include "dq_device.hpp"
 
 namespace dq {
-
-enum TileKind : int {
-    T_DENSE1 = 0,    // 2x2 on register slot p (diagonal and anti-diagonal included)
-    T_CTRL1 = 1,     // 2x2 on register slot p where (tile index & cmask) == cval
-    T_DENSE2 = 2,    // 4x4 on register slots p (matrix MSB) and q
-    T_DIAG = 3,      // diagonal 2^k on any tile bits, elementwise
-    T_COLLAPSE = 4,  // measurement collapse on tile bit bit[0], elementwise
-    T_LDSOP = 5      // any 2^k x 2^k matrix through LDS, one output per amplitude
-};
-
-// Product shape of a matrix entry, 3 bits per entry in TileOp::modes.
-enum EntryMode : int { M_ZERO = 0, M_ONE = 1, M_MONE = 2, M_REAL = 3, M_IMAG = 4, M_GEN = 5 };
-
-struct TileOp {
-    int kind;
-    int k;
-    int relayout;  // 1: change to layout `rb` before this op
-    int rb;        // layout active from this op: 4 tile bits, 4 bits each, ascending
-    int p, q;      // register slots (see TileKind)
-    int bit[5];    // tile bits, bit[0] = MSB of the gate index
-    int sbit[5];   // ascending
-    unsigned cmask, cval;
-    int mat;       // offset (complex entries) into the matrix pool
-    int outcome;
-    unsigned long long modes;
-    double norm;
-};
-
-struct TilePass {
-    int n_outer;
-    int op_begin;
-    int op_count;
-    int load_rb;   // layout of the global load
-    int store_rb;  // layout of the global store
-    int tq[16];    // tile bit -> qubit
-    int oq[48];    // outer bit -> qubit
-};
 
 template <typename T>
 struct Layout {
@@ -108,183 +72,172 @@ __device__ __forceinline__ uint64_t gmap(unsigned i, const TilePass &pd) {
     return g;
 }
 
-// ---- exact products by entry shape ----
+// ---- register ops ----
 
-template <typename T, int N>
-__device__ __forceinline__ void prod_n(cplx<T> (&out)[N], const cplx<T> (&v)[N], cplx<T> u, int mode) {
-    switch (mode) {
-    case M_ONE:
-#pragma unroll
-        for (int i = 0; i < N; ++i) out[i] = v[i];
-        break;
-    case M_MONE:
-#pragma unroll
-        for (int i = 0; i < N; ++i) out[i] = cplx<T>{-v[i].re, -v[i].im};
-        break;
-    case M_REAL:
-#pragma unroll
-        for (int i = 0; i < N; ++i) out[i] = cplx<T>{u.re * v[i].re, u.re * v[i].im};
-        break;
-    case M_IMAG:
-#pragma unroll
-        for (int i = 0; i < N; ++i) out[i] = cplx<T>{-(u.im * v[i].im), u.im * v[i].re};
-        break;
-    default:
-#pragma unroll
-        for (int i = 0; i < N; ++i) out[i] = cmul(u, v[i]);
-        break;
-    }
+template <typename T>
+__device__ __forceinline__ cplx<T> rmul(cplx<T> u, cplx<T> v) {  // u real: u.im products dropped
+    return cplx<T>{u.re * v.re, u.re * v.im};
 }
 
-// acc (+)= u * v with the reference's order; `first` means acc holds 0.
-template <typename T, int N>
-__device__ __forceinline__ void mac_n(cplx<T> (&acc)[N], bool &first, const cplx<T> (&v)[N], cplx<T> u, int mode) {
-    if (mode == M_ZERO) return;
-    if (first) {
-        prod_n<T, N>(acc, v, u, mode);
-        first = false;
-        return;
-    }
-    cplx<T> p[N];
-    prod_n<T, N>(p, v, u, mode);
-#pragma unroll
-    for (int i = 0; i < N; ++i) acc[i] = cadd(acc[i], p[i]);
+// Per-lane choice among four uniform values, as two levels of selects on the
+// bits of e (a select chain on e's value can be lowered to a stack table).
+template <typename T>
+__device__ __forceinline__ T sel(bool c, T a, T b) {
+    return c ? a : b;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> pick4(int e, cplx<T> m0, cplx<T> m1, cplx<T> m2, cplx<T> m3) {
+    const bool b0 = (e & 1) != 0, b1 = (e & 2) != 0;
+    const T lo_re = sel(b0, m1.re, m0.re), lo_im = sel(b0, m1.im, m0.im);
+    const T hi_re = sel(b0, m3.re, m2.re), hi_im = sel(b0, m3.im, m2.im);
+    return cplx<T>{sel(b1, hi_re, lo_re), sel(b1, hi_im, lo_im)};
 }
 
-template <typename T, int N>
-__device__ __forceinline__ void zero_if_first(cplx<T> (&acc)[N], bool first) {
-    if (first) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) acc[i] = cplx<T>{T(0), T(0)};
-    }
+template <int P>
+__device__ __forceinline__ constexpr int pair_lo(int jj) {
+    return ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
 }
 
-__device__ __forceinline__ int emode(unsigned long long modes, int e) { return int((modes >> (3 * e)) & 7); }
-
-// 2x2 on register slot P: pairs (j0, j0 | 1 << P).
-template <typename T, int P>
-__device__ __forceinline__ void reg_dense1(cplx<T> (&a)[16], const cplx<T> *__restrict__ m, unsigned long long modes) {
-    cplx<T> x[8], y[8];
+// 2x2 on register slot P, pairs (j0, j0 | 1 << P); out = u00*x + u01*y, u10*x + u11*y.
+template <typename T, int P, int V>
+__device__ __forceinline__ void reg_dense1(cplx<T> (&a)[16], const cplx<T> *__restrict__ m) {
+    const cplx<T> u00 = m[0], u01 = m[1], u10 = m[2], u11 = m[3];
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
-        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
-        x[jj] = a[j0];
-        y[jj] = a[j0 | (1 << P)];
-    }
-    cplx<T> r0[8], r1[8];
-    bool f0 = true, f1 = true;
-    mac_n<T, 8>(r0, f0, x, m[0], emode(modes, 0));
-    mac_n<T, 8>(r0, f0, y, m[1], emode(modes, 1));
-    zero_if_first<T, 8>(r0, f0);
-    mac_n<T, 8>(r1, f1, x, m[2], emode(modes, 2));
-    mac_n<T, 8>(r1, f1, y, m[3], emode(modes, 3));
-    zero_if_first<T, 8>(r1, f1);
-#pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
-        a[j0] = r0[jj];
-        a[j0 | (1 << P)] = r1[jj];
-    }
-}
-
-// Controlled 2x2 on slot P: pair applied only where the control pattern holds.
-template <typename T, int P>
-__device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[16], const cplx<T> *__restrict__ m, unsigned long long modes,
-                                          const Layout<T> &L, unsigned cmask, unsigned cval) {
-    cplx<T> x[8], y[8];
-#pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
-        x[jj] = a[j0];
-        y[jj] = a[j0 | (1 << P)];
-    }
-    cplx<T> r0[8], r1[8];
-    bool f0 = true, f1 = true;
-    mac_n<T, 8>(r0, f0, x, m[0], emode(modes, 0));
-    mac_n<T, 8>(r0, f0, y, m[1], emode(modes, 1));
-    zero_if_first<T, 8>(r0, f0);
-    mac_n<T, 8>(r1, f1, x, m[2], emode(modes, 2));
-    mac_n<T, 8>(r1, f1, y, m[3], emode(modes, 3));
-    zero_if_first<T, 8>(r1, f1);
-#pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
-        const int j0 = ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
-        const bool on = ((L.tpart | L.jp[j0]) & cmask) == cval;
-        if (on) {
-            a[j0] = r0[jj];
-            a[j0 | (1 << P)] = r1[jj];
+        const int j0 = pair_lo<P>(jj), j1 = j0 | (1 << P);
+        const cplx<T> x = a[j0], y = a[j1];
+        if (V == V_GEN) {
+            a[j0] = cadd(cmul(u00, x), cmul(u01, y));
+            a[j1] = cadd(cmul(u10, x), cmul(u11, y));
+        } else if (V == V_REAL) {
+            a[j0] = cadd(rmul(u00, x), rmul(u01, y));
+            a[j1] = cadd(rmul(u10, x), rmul(u11, y));
+        } else if (V == V_DIAG) {
+            a[j0] = cmul(u00, x);
+            a[j1] = cmul(u11, y);
+        } else {
+            a[j0] = cmul(u01, y);
+            a[j1] = cmul(u10, x);
         }
     }
 }
 
-// 4x4 on register slots P (matrix MSB) and Q.
-template <typename T, int P, int Q>
-__device__ __forceinline__ void reg_dense2(cplx<T> (&a)[16], const cplx<T> *__restrict__ m, unsigned long long modes) {
+// Controlled 2x2 on slot P: the pair changes only where the control pattern holds.
+template <typename T, int P, int V>
+__device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[16], const cplx<T> *__restrict__ m, const Layout<T> &L,
+                                          unsigned cmask, unsigned cval) {
+    const cplx<T> u00 = m[0], u01 = m[1], u10 = m[2], u11 = m[3];
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+        const int j0 = pair_lo<P>(jj), j1 = j0 | (1 << P);
+        const cplx<T> x = a[j0], y = a[j1];
+        cplx<T> n0, n1;
+        if (V == V_ANTI) {
+            n0 = cmul(u01, y);
+            n1 = cmul(u10, x);
+        } else {
+            n0 = cadd(cmul(u00, x), cmul(u01, y));
+            n1 = cadd(cmul(u10, x), cmul(u11, y));
+        }
+        const bool on = ((L.tpart | L.jp[j0]) & cmask) == cval;
+        a[j0] = on ? n0 : x;
+        a[j1] = on ? n1 : y;
+    }
+}
+
+// 4x4 on register slots P (matrix MSB) and Q: acc = 0; acc += U[r][c] * v[c].
+template <typename T, int P, int Q, int V>
+__device__ __forceinline__ void reg_dense2(cplx<T> (&a)[16], const cplx<T> *__restrict__ m) {
     constexpr int LO = P < Q ? P : Q, HI = P < Q ? Q : P;
-    int base[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-        int v = g;
-        v = ((v >> LO) << (LO + 1)) | (v & ((1 << LO) - 1));
-        v = ((v >> HI) << (HI + 1)) | (v & ((1 << HI) - 1));
-        base[g] = v;
+        int base = g;
+        base = ((base >> LO) << (LO + 1)) | (base & ((1 << LO) - 1));
+        base = ((base >> HI) << (HI + 1)) | (base & ((1 << HI) - 1));
+        cplx<T> v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = a[base | (((c >> 1) & 1) << P) | ((c & 1) << Q)];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cplx<T> acc;
+            if (V == V_DIAG) {
+                acc = cmul(m[r * 5], v[r]);
+            } else if (V == V_REAL) {
+                acc = rmul(m[r * 4], v[0]);
+#pragma unroll
+                for (int c = 1; c < 4; ++c) acc = cadd(acc, rmul(m[r * 4 + c], v[c]));
+            } else {
+                acc = cmul(m[r * 4], v[0]);
+#pragma unroll
+                for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(m[r * 4 + c], v[c]));
+            }
+            a[base | (((r >> 1) & 1) << P) | ((r & 1) << Q)] = acc;
+        }
     }
-    cplx<T> v[4][4];  // v[c][g]
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int g = 0; g < 4; ++g) v[c][g] = a[base[g] | (((c >> 1) & 1) << P) | ((c & 1) << Q)];
-    cplx<T> out[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        bool f = true;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) mac_n<T, 4>(out[r], f, v[c], m[r * 4 + c], emode(modes, r * 4 + c));
-        zero_if_first<T, 4>(out[r], f);
+}
+
+template <typename T, int P>
+__device__ __forceinline__ void dense1_v(cplx<T> (&a)[16], int v, const cplx<T> *m) {
+    switch (v) {
+    case V_REAL: reg_dense1<T, P, V_REAL>(a, m); break;
+    case V_DIAG: reg_dense1<T, P, V_DIAG>(a, m); break;
+    case V_ANTI: reg_dense1<T, P, V_ANTI>(a, m); break;
+    default: reg_dense1<T, P, V_GEN>(a, m); break;
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int g = 0; g < 4; ++g) a[base[g] | (((r >> 1) & 1) << P) | ((r & 1) << Q)] = out[r][g];
 }
 
 template <typename T>
-__device__ __forceinline__ void dispatch_dense1(cplx<T> (&a)[16], int p, const cplx<T> *m, unsigned long long modes) {
+__device__ __forceinline__ void dispatch_dense1(cplx<T> (&a)[16], int p, int v, const cplx<T> *m) {
     switch (p) {
-    case 0: reg_dense1<T, 0>(a, m, modes); break;
-    case 1: reg_dense1<T, 1>(a, m, modes); break;
-    case 2: reg_dense1<T, 2>(a, m, modes); break;
-    default: reg_dense1<T, 3>(a, m, modes); break;
+    case 0: dense1_v<T, 0>(a, v, m); break;
+    case 1: dense1_v<T, 1>(a, v, m); break;
+    case 2: dense1_v<T, 2>(a, v, m); break;
+    default: dense1_v<T, 3>(a, v, m); break;
     }
 }
 
+template <typename T, int P>
+__device__ __forceinline__ void ctrl1_v(cplx<T> (&a)[16], int v, const cplx<T> *m, const Layout<T> &L, unsigned cm,
+                                        unsigned cv) {
+    if (v == V_ANTI) reg_ctrl1<T, P, V_ANTI>(a, m, L, cm, cv);
+    else reg_ctrl1<T, P, V_GEN>(a, m, L, cm, cv);
+}
+
 template <typename T>
-__device__ __forceinline__ void dispatch_ctrl1(cplx<T> (&a)[16], int p, const cplx<T> *m, unsigned long long modes,
-                                               const Layout<T> &L, unsigned cmask, unsigned cval) {
+__device__ __forceinline__ void dispatch_ctrl1(cplx<T> (&a)[16], int p, int v, const cplx<T> *m, const Layout<T> &L,
+                                               unsigned cmask, unsigned cval) {
     switch (p) {
-    case 0: reg_ctrl1<T, 0>(a, m, modes, L, cmask, cval); break;
-    case 1: reg_ctrl1<T, 1>(a, m, modes, L, cmask, cval); break;
-    case 2: reg_ctrl1<T, 2>(a, m, modes, L, cmask, cval); break;
-    default: reg_ctrl1<T, 3>(a, m, modes, L, cmask, cval); break;
+    case 0: ctrl1_v<T, 0>(a, v, m, L, cmask, cval); break;
+    case 1: ctrl1_v<T, 1>(a, v, m, L, cmask, cval); break;
+    case 2: ctrl1_v<T, 2>(a, v, m, L, cmask, cval); break;
+    default: ctrl1_v<T, 3>(a, v, m, L, cmask, cval); break;
+    }
+}
+
+template <typename T, int P, int Q>
+__device__ __forceinline__ void dense2_v(cplx<T> (&a)[16], int v, const cplx<T> *m) {
+    switch (v) {
+    case V_REAL: reg_dense2<T, P, Q, V_REAL>(a, m); break;
+    case V_DIAG: reg_dense2<T, P, Q, V_DIAG>(a, m); break;
+    default: reg_dense2<T, P, Q, V_GEN>(a, m); break;
     }
 }
 
 template <typename T>
-__device__ __forceinline__ void dispatch_dense2(cplx<T> (&a)[16], int p, int q, const cplx<T> *m,
-                                                unsigned long long modes) {
+__device__ __forceinline__ void dispatch_dense2(cplx<T> (&a)[16], int p, int q, int v, const cplx<T> *m) {
     switch (p * 4 + q) {
-    case 0 * 4 + 1: reg_dense2<T, 0, 1>(a, m, modes); break;
-    case 0 * 4 + 2: reg_dense2<T, 0, 2>(a, m, modes); break;
-    case 0 * 4 + 3: reg_dense2<T, 0, 3>(a, m, modes); break;
-    case 1 * 4 + 0: reg_dense2<T, 1, 0>(a, m, modes); break;
-    case 1 * 4 + 2: reg_dense2<T, 1, 2>(a, m, modes); break;
-    case 1 * 4 + 3: reg_dense2<T, 1, 3>(a, m, modes); break;
-    case 2 * 4 + 0: reg_dense2<T, 2, 0>(a, m, modes); break;
-    case 2 * 4 + 1: reg_dense2<T, 2, 1>(a, m, modes); break;
-    case 2 * 4 + 3: reg_dense2<T, 2, 3>(a, m, modes); break;
-    case 3 * 4 + 0: reg_dense2<T, 3, 0>(a, m, modes); break;
-    case 3 * 4 + 1: reg_dense2<T, 3, 1>(a, m, modes); break;
-    default: reg_dense2<T, 3, 2>(a, m, modes); break;
+    case 0 * 4 + 1: dense2_v<T, 0, 1>(a, v, m); break;
+    case 0 * 4 + 2: dense2_v<T, 0, 2>(a, v, m); break;
+    case 0 * 4 + 3: dense2_v<T, 0, 3>(a, v, m); break;
+    case 1 * 4 + 0: dense2_v<T, 1, 0>(a, v, m); break;
+    case 1 * 4 + 2: dense2_v<T, 1, 2>(a, v, m); break;
+    case 1 * 4 + 3: dense2_v<T, 1, 3>(a, v, m); break;
+    case 2 * 4 + 0: dense2_v<T, 2, 0>(a, v, m); break;
+    case 2 * 4 + 1: dense2_v<T, 2, 1>(a, v, m); break;
+    case 2 * 4 + 3: dense2_v<T, 2, 3>(a, v, m); break;
+    case 3 * 4 + 0: dense2_v<T, 3, 0>(a, v, m); break;
+    case 3 * 4 + 1: dense2_v<T, 3, 1>(a, v, m); break;
+    default: dense2_v<T, 3, 2>(a, v, m); break;
     }
 }
 
@@ -301,6 +254,8 @@ __device__ __forceinline__ void from_lds(cplx<T> (&a)[16], const cplx<T> *lds, c
 }
 
 // One HBM sweep with register-resident amplitudes (see the file header).
+// The next tile's amplitudes are loaded into registers before the current
+// tile's ops run, so each tile's HBM latency overlaps the previous tile's work.
 template <typename T, int B>
 __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__ amps, TilePass pd,
                                                            const TileOp *__restrict__ ops,
@@ -308,27 +263,59 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
     const unsigned t = threadIdx.x;
-    Layout<T> L0, L;
-    make_layout<T, B>(L0, pd.load_rb, t);
-    const uint64_t g_t = gmap<B>(L0.tpart, pd);
-    uint64_t g_j[16];
+    Layout<T> L;
+    // global offsets of the load/store layouts: per-thread part + 4 uniform
+    // per-register-bit offsets (additive over disjoint bits)
+    uint64_t g_t, gs_t, g_r[4], gs_r[4];
+    {
+        Layout<T> L0;
+        make_layout<T, B>(L0, pd.load_rb, t);
+        g_t = gmap<B>(L0.tpart, pd);
+        make_layout<T, B>(L0, pd.store_rb, t);
+        gs_t = gmap<B>(L0.tpart, pd);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) g_j[j] = gmap<B>(L0.jp[j], pd);
-    Layout<T> LS;
-    make_layout<T, B>(LS, pd.store_rb, t);
-    const uint64_t gs_t = gmap<B>(LS.tpart, pd);
-    uint64_t gs_j[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) gs_j[j] = gmap<B>(LS.jp[j], pd);
-
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        for (int mm = 0; mm < 4; ++mm) {
+            g_r[mm] = uint64_t(1) << pd.tq[(pd.load_rb >> (4 * mm)) & 15];
+            gs_r[mm] = uint64_t(1) << pd.tq[(pd.store_rb >> (4 * mm)) & 15];
+        }
+    }
+    auto tile_base = [&](uint64_t tile) {
         uint64_t base = 0;
         for (int j = 0; j < pd.n_outer; ++j)
             if ((tile >> j) & 1) base |= uint64_t(1) << pd.oq[j];
+        return base;
+    };
+    cplx<T> nxt[16];
+    uint64_t tile = blockIdx.x;
+    if (tile < ntiles) {
+        const uint64_t base = tile_base(tile) + g_t;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            uint64_t off = base;
+#pragma unroll
+            for (int mm = 0; mm < 4; ++mm)
+                if ((j >> mm) & 1) off += g_r[mm];
+            nxt[j] = amps[off];
+        }
+    }
+    for (; tile < ntiles; tile += gridDim.x) {
+        const uint64_t base = tile_base(tile);
         cplx<T> a[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) a[j] = amps[base + g_t + g_j[j]];
-        L = L0;
+        for (int j = 0; j < 16; ++j) a[j] = nxt[j];
+        const uint64_t next = tile + gridDim.x;
+        if (next < ntiles) {
+            const uint64_t nb = tile_base(next) + g_t;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                uint64_t off = nb;
+#pragma unroll
+                for (int mm = 0; mm < 4; ++mm)
+                    if ((j >> mm) & 1) off += g_r[mm];
+                nxt[j] = amps[off];
+            }
+        }
+        make_layout<T, B>(L, pd.load_rb, t);
         int cur = pd.load_rb;
         for (int o = 0; o < pd.op_count; ++o) {
             const TileOp &op = ops[pd.op_begin + o];
@@ -343,14 +330,34 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             const cplx<T> *m = mats + op.mat;
             switch (op.kind) {
             case T_DENSE1:
-                dispatch_dense1<T>(a, op.p, m, op.modes);
+                dispatch_dense1<T>(a, op.p, op.variant, m);
                 break;
             case T_CTRL1:
-                dispatch_ctrl1<T>(a, op.p, m, op.modes, L, op.cmask, op.cval);
+                dispatch_ctrl1<T>(a, op.p, op.variant, m, L, op.cmask, op.cval);
                 break;
             case T_DENSE2:
-                dispatch_dense2<T>(a, op.p, op.q, m, op.modes);
+                dispatch_dense2<T>(a, op.p, op.q, op.variant, m);
                 break;
+            case T_DIAGT: {
+                // e = thread part | register part; the thread part picks the
+                // candidate entries once per op (uniform values, no gathers).
+                int et = 0;
+                for (int b2 = 0; b2 < op.k; ++b2)
+                    if (b2 != op.q || op.p < 0) et |= int((L.tpart >> op.bit[b2]) & 1) << (op.k - 1 - b2);
+                const cplx<T> m0 = m[0], m1 = m[1];
+                const cplx<T> m2 = op.k == 2 ? m[2] : m0, m3 = op.k == 2 ? m[3] : m1;
+                if (op.p < 0) {
+                    const cplx<T> d = pick4(et, m0, m1, m2, m3);
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) a[j] = cmul(d, a[j]);
+                } else {
+                    const int rw = 1 << (op.k - 1 - op.q);  // weight of the register bit in e
+                    const cplx<T> d0 = pick4(et, m0, m1, m2, m3), d1 = pick4(et | rw, m0, m1, m2, m3);
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) a[j] = cmul(((j >> op.p) & 1) ? d1 : d0, a[j]);
+                }
+                break;
+            }
             case T_DIAG:
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
@@ -389,16 +396,12 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                     for (int b2 = 0; b2 < k; ++b2) r = (r << 1) | int((i >> op.bit[b2]) & 1);
                     const unsigned base_i = i & ~gm;
                     cplx<T> acc{T(0), T(0)};
-                    bool first = true;
                     for (int c = 0; c < dim; ++c) {
-                        const cplx<T> u = m[r * dim + c];
-                        if (u.re == T(0) && u.im == T(0)) continue;
                         unsigned off = 0;
                         for (int b2 = 0; b2 < k; ++b2)
                             if ((c >> (k - 1 - b2)) & 1) off |= 1u << op.bit[b2];
-                        const cplx<T> pr = cmul(u, lds[swz<B>(base_i | off)]);
-                        acc = first ? pr : cadd(acc, pr);
-                        first = false;
+                        const cplx<T> pr = cmul(m[r * dim + c], lds[swz<B>(base_i | off)]);
+                        acc = (c == 0) ? pr : cadd(acc, pr);
                     }
                     a[j] = acc;
                 }
@@ -410,10 +413,18 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             __syncthreads();
             to_lds<T, B>(lds, a, L);
             __syncthreads();
-            from_lds<T, B>(a, lds, LS);
+            make_layout<T, B>(L, pd.store_rb, t);
+            from_lds<T, B>(a, lds, L);
         }
+        const uint64_t sb = base + gs_t;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) amps[base + gs_t + gs_j[j]] = a[j];
+        for (int j = 0; j < 16; ++j) {
+            uint64_t off = sb;
+#pragma unroll
+            for (int mm = 0; mm < 4; ++mm)
+                if ((j >> mm) & 1) off += gs_r[mm];
+            amps[off] = a[j];
+        }
         // The next tile's first LDS write is preceded by a barrier.
     }
 }

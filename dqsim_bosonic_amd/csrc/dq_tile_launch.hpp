@@ -7,9 +7,6 @@
 
 #include "dq_tile.hpp"
 
-// Tile sizes with a compiled kernel; any other returns hipErrorInvalidValue.
-inline bool dq_tile_bits_supported(int b) { return b == 9 || b == 12 || b == 13; }
-
 hipError_t dq_launch_tile_reg_f64(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
                                   const dq::TileOp *ops, const void *mats, uint64_t ntiles);
 hipError_t dq_launch_tile_reg_f32(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,

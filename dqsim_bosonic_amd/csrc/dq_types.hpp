@@ -1,0 +1,73 @@
+// dq_types.hpp -- plain data shared by the host planner and the kernels.
+#pragma once
+#include <stdint.h>
+
+namespace dq {
+
+enum OpKind : int {
+    K_DENSE1 = 0,   // 2x2 on bit[0]
+    K_DIAG = 1,     // diagonal 2^k, entries at mat
+    K_MONO = 2,     // monomial: out[r] = ph[r] * in[perm[r]], ph at mat, perm at aux
+    K_CTRL1 = 3,    // 2x2 on bit[k-1] where (i & cmask) == cval
+    K_DENSEK = 4,   // dense 2^k x 2^k, acc = 0; acc += U[r][c] * v[c]
+    K_COLLAPSE = 5  // keep bit[0] == outcome (divide by norm), zero the rest
+};
+
+// One queued operation, positions relative to the index space it runs on
+// (global qubits for direct launches, tile-local bits inside a tile pass).
+struct DevOp {
+    int kind;
+    int k;
+    int mat;          // offset (complex entries) into the matrix pool
+    int aux;          // offset (ints) into the perm pool (K_MONO)
+    int bit[5];       // bit[0] = MSB of the gate index
+    int sbit[5];      // the same positions, ascending
+    int outcome;
+    int pad;
+    uint64_t cmask;   // K_CTRL1
+    uint64_t cval;
+    double norm;      // K_COLLAPSE
+};
+
+enum TileKind : int {
+    T_DENSE1 = 0,    // 2x2 on register slot p; variant: V_GEN / V_REAL / V_DIAG / V_ANTI
+    T_CTRL1 = 1,     // 2x2 on register slot p where (tile index & cmask) == cval; V_GEN / V_ANTI
+    T_DENSE2 = 2,    // 4x4 on register slots p (matrix MSB) and q; V_GEN / V_REAL / V_DIAG
+    T_DIAG = 3,      // diagonal 2^k on any tile bits, elementwise
+    T_COLLAPSE = 4,  // measurement collapse on tile bit bit[0], elementwise
+    T_LDSOP = 5,     // any 2^k x 2^k matrix through LDS, one output per amplitude
+    T_DIAGT = 6      // diagonal, k <= 2, at most one register bit (slot p, gate-index bit q)
+};
+
+// Matrix shape of a register op, chosen on the host.  Every variant computes
+// the reference's row sums term by term; V_REAL drops only the products of
+// zero imaginary parts, V_DIAG / V_ANTI only the products of zero entries.
+enum OpVariant : int { V_GEN = 0, V_REAL = 1, V_DIAG = 2, V_ANTI = 3 };
+
+struct TileOp {
+    int kind;
+    int k;
+    int relayout;  // 1: change to layout `rb` before this op
+    int rb;        // layout active from this op: 4 tile bits, 4 bits each, ascending
+    int p, q;      // register slots (see TileKind)
+    int bit[5];    // tile bits, bit[0] = MSB of the gate index
+    int sbit[5];   // ascending
+    unsigned cmask, cval;
+    int mat;       // offset (complex entries) into the matrix pool
+    int outcome;
+    int variant;
+    int pad;
+    double norm;
+};
+
+struct TilePass {
+    int n_outer;
+    int op_begin;
+    int op_count;
+    int load_rb;   // layout of the global load
+    int store_rb;  // layout of the global store
+    int tq[16];    // tile bit -> qubit
+    int oq[48];    // outer bit -> qubit
+};
+
+}  // namespace dq
