@@ -197,6 +197,44 @@ __global__ __launch_bounds__(kThreads) void k_prob_one(const cplx<T> *__restrict
     if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
+// p1 of the state that the queued elementwise ops (diagonals, collapses)
+// would produce, without writing it: the same grid-stride order and block
+// sums as k_prob_one, each element computed with the ops' own arithmetic, so
+// the result equals k_prob_one on the updated state.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_prob_one_virtual(const cplx<T> *__restrict__ amps, int n, int q,
+                                                               const DevOp *__restrict__ ops, int nops,
+                                                               const cplx<T> *__restrict__ mats,
+                                                               double *__restrict__ partials) {
+    __shared__ double scratch[kThreads / 64];
+    const uint64_t half = uint64_t(1) << q;
+    const uint64_t count = uint64_t(1) << (n - 1);
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    double s = 0.0;
+    for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < count; g += stride) {
+        const uint64_t i = ((g >> q) << (q + 1)) | half | (g & (half - 1));
+        cplx<T> a = amps[i];
+        for (int o = 0; o < nops; ++o) {
+            const DevOp &op = ops[o];
+            if (op.kind == K_DIAG) {
+                a = cmul(mats[op.mat + gather_bits<uint64_t>(i, op.bit, op.k)], a);
+            } else {  // K_COLLAPSE
+                if (int((i >> op.bit[0]) & 1) == op.outcome) {
+                    const T nrm = T(op.norm);
+                    a.re = a.re / nrm;
+                    a.im = a.im / nrm;
+                } else {
+                    a = cplx<T>{T(0), T(0)};
+                }
+            }
+        }
+        const double re = double(a.re), im = double(a.im);
+        s += re * re + im * im;
+    }
+    const double r = block_sum(s, scratch);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
 // Sums `count` partials (one block, fixed order) -> out[slot].
 __global__ __launch_bounds__(kThreads) void k_sum_partials(const double *__restrict__ partials, int count,
                                                            int stride_elems, double *__restrict__ out) {
@@ -328,6 +366,7 @@ struct dq_state {
     int fuse = 2;
     int tile_bits = 12;
     int reg_tiles = 1;  // register-resident tile kernel for tiles of >= 9 qubits
+    int last_grid = 0;  // grid of the last register-tile launch (partials it wrote)
     int low_bits = 5;
     std::vector<HostOp> queue;
     // upload buffers
@@ -570,8 +609,11 @@ int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *ma
     const uint64_t ntiles = uint64_t(1) << rp.pd.n_outer;
     const uint64_t grid = std::min<uint64_t>(ntiles, 16384);
     hipError_t e = std::is_same<T, double>::value
-                       ? dq_launch_tile_reg_f64(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles)
-                       : dq_launch_tile_reg_f32(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles);
+                       ? dq_launch_tile_reg_f64(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles,
+                                                s->d_partials)
+                       : dq_launch_tile_reg_f32(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles,
+                                                s->d_partials);
+    s->last_grid = int(grid);
     if (e == hipErrorInvalidValue) return fail(DQ_ERR_ARG, "unsupported tile size " + std::to_string(rp.b));
     HIP_TRY(e);
     s->launches++;
@@ -579,8 +621,10 @@ int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *ma
     return DQ_OK;
 }
 
+// red_qubit >= 0: the last pass also sums |amp|^2 over that qubit being 1
+// into per-block partials (s->last_grid of them).
 template <typename T>
-int flush_reg(dq_state *s, const std::vector<Pass> &passes) {
+int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) {
     const PlanConfig cfg = plan_config(s);
     std::vector<RegPlan> plans(passes.size());
     std::vector<double> mats;
@@ -589,6 +633,17 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes) {
         plan_pass(&cfg, s->queue, passes[i], mats, plans[i]);
         plans[i].pd.op_begin = int(total_ops);
         plans[i].pd.op_count = int(plans[i].ops.size());
+        plans[i].pd.red_bit = -1;
+        if (red_qubit >= 0 && i + 1 == passes.size()) {
+            int ob = 0;
+            for (int q = 0; q < s->n; ++q) {
+                if ((passes[i].S >> q) & 1) continue;
+                if (q == red_qubit) plans[i].pd.red_bit = kOuterBit + ob;
+                ++ob;
+            }
+            for (int tb = 0; tb < plans[i].b; ++tb)
+                if (plans[i].pd.tq[tb] == red_qubit) plans[i].pd.red_bit = tb;
+        }
         total_ops += plans[i].ops.size();
     }
     const size_t real_b = sizeof(T);
@@ -617,7 +672,7 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes) {
 }
 
 template <typename T>
-int flush_t(dq_state *s) {
+int flush_t(dq_state *s, int red_qubit = -1) {
     if (s->queue.empty()) return DQ_OK;
     const int n = s->n;
     Packed pk;
@@ -625,7 +680,7 @@ int flush_t(dq_state *s) {
     int rc = DQ_OK;
     const PlanConfig cfg = plan_config(s);
     if (s->fuse && s->reg_tiles && effective_tile_bits(&cfg) >= 9) {
-        return flush_reg<T>(s, schedule(&cfg, s->queue));
+        return flush_reg<T>(s, schedule(&cfg, s->queue, true), red_qubit);
     }
     if (s->fuse && n >= 1) {
         std::vector<Pass> passes = schedule(&cfg, s->queue);
@@ -677,6 +732,18 @@ int flush_t(dq_state *s) {
 
 int flush(dq_state *s) {
     return s->dtype == DQ_C128 ? flush_t<double>(s) : flush_t<float>(s);
+}
+
+// True when the register-tile path handles the flush (it can fuse a reduction).
+bool reg_path(const dq_state *s) {
+    const PlanConfig cfg = plan_config(s);
+    return s->fuse && s->reg_tiles && effective_tile_bits(&cfg) >= 9;
+}
+
+bool all_elementwise(const dq_state *s) {
+    for (const HostOp &h : s->queue)
+        if (h.kind != K_DIAG && h.kind != K_COLLAPSE) return false;
+    return true;
 }
 
 int enqueue(dq_state *s, HostOp &&op) {
@@ -876,19 +943,48 @@ int dq_collapse(dq_state *s, int qubit, int outcome, double norm) {
 int dq_prob_one(dq_state *s, int qubit, double *p1) {
     if (!s || !p1) return fail(DQ_ERR_ARG, "null argument");
     if (int rc = check_qubit(s, qubit)) return rc;
-    if (int rc = flush(s)) return rc;
     const uint64_t count = amp_count(s) / 2;
     const int blocks = std::min<int>(kReduceBlocks, grid_for(count));
-    if (s->dtype == DQ_C128)
-        hipLaunchKernelGGL(k_prob_one<double>, dim3(blocks), dim3(kThreads), 0, s->stream,
-                           static_cast<const cplx<double> *>(s->amps), s->n, qubit, s->d_partials);
-    else
-        hipLaunchKernelGGL(k_prob_one<float>, dim3(blocks), dim3(kThreads), 0, s->stream,
-                           static_cast<const cplx<float> *>(s->amps), s->n, qubit, s->d_partials);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, s->stream, s->d_partials, blocks, 1,
+    int nparts = blocks;
+    if (!s->queue.empty() && s->queue.size() <= 64 && all_elementwise(s)) {
+        // only diagonals/collapses pending: reduce over the state they would
+        // produce and keep them queued for the next sweep
+        Packed pk;
+        int pos[64];
+        for (int q = 0; q < 64; ++q) pos[q] = q;
+        for (const HostOp &h : s->queue) pk.ops.push_back(to_dev(h, pos, pk));
+        DevOp *d_ops;
+        void *d_mats;
+        int *d_perms;
+        if (int rc = upload(s, pk, &d_ops, &d_mats, &d_perms)) return rc;
+        if (s->dtype == DQ_C128)
+            hipLaunchKernelGGL(k_prob_one_virtual<double>, dim3(blocks), dim3(kThreads), 0, s->stream,
+                               static_cast<const cplx<double> *>(s->amps), s->n, qubit, d_ops, int(pk.ops.size()),
+                               static_cast<const cplx<double> *>(d_mats), s->d_partials);
+        else
+            hipLaunchKernelGGL(k_prob_one_virtual<float>, dim3(blocks), dim3(kThreads), 0, s->stream,
+                               static_cast<const cplx<float> *>(s->amps), s->n, qubit, d_ops, int(pk.ops.size()),
+                               static_cast<const cplx<float> *>(d_mats), s->d_partials);
+        s->launches++;
+    } else if (!s->queue.empty() && reg_path(s)) {
+        // the flush's last sweep sums p1 as it writes the state
+        const int rc = s->dtype == DQ_C128 ? flush_t<double>(s, qubit) : flush_t<float>(s, qubit);
+        if (rc) return rc;
+        nparts = s->last_grid;
+    } else {
+        if (int rc = flush(s)) return rc;
+        if (s->dtype == DQ_C128)
+            hipLaunchKernelGGL(k_prob_one<double>, dim3(blocks), dim3(kThreads), 0, s->stream,
+                               static_cast<const cplx<double> *>(s->amps), s->n, qubit, s->d_partials);
+        else
+            hipLaunchKernelGGL(k_prob_one<float>, dim3(blocks), dim3(kThreads), 0, s->stream,
+                               static_cast<const cplx<float> *>(s->amps), s->n, qubit, s->d_partials);
+        s->launches++;
+    }
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, s->stream, s->d_partials, nparts, 1,
                        s->d_result);
     HIP_TRY(hipGetLastError());
-    s->launches += 2;
+    s->launches++;
     if (int rc = fetch_result(s, 1)) return rc;
     *p1 = s->h_result[0];
     return DQ_OK;

@@ -133,7 +133,22 @@ inline int effective_tile_bits(const PlanConfig *s) {
     return b;
 }
 
-inline std::vector<Pass> schedule(const PlanConfig *s, const std::vector<HostOp> &queue) {
+// Qubits an op needs inside the tile.  With `anywhere`, diagonal ops and
+// collapses need none (an outer qubit's bit is constant over a tile) and a
+// controlled 2x2 needs only its target.
+inline uint64_t tile_need(const HostOp &h, bool anywhere) {
+    if (!anywhere) return h.qmask();
+    if (h.kind == K_DIAG || h.kind == K_COLLAPSE) return 0;
+    if (h.kind == K_CTRL1) return uint64_t(1) << h.q[h.k - 1];
+    return h.qmask();
+}
+
+inline bool is_diagonal(const HostOp &h) { return h.kind == K_DIAG || h.kind == K_COLLAPSE; }
+
+// Greedy pass builder (see dq_engine.hip).  `anywhere` (register tiles only):
+// diagonal ops may name outer qubits, and at fusion level 2 they may move
+// past other diagonal ops (diagonal matrices commute).
+inline std::vector<Pass> schedule(const PlanConfig *s, const std::vector<HostOp> &queue, bool anywhere = false) {
     std::vector<Pass> passes;
     const int n = s->n;
     const int b = effective_tile_bits(s);
@@ -145,23 +160,27 @@ inline std::vector<Pass> schedule(const PlanConfig *s, const std::vector<HostOp>
     while (!remaining.empty()) {
         Pass p;
         p.S = low_mask;
-        uint64_t blocked = 0;
+        uint64_t nd_block = 0, d_block = 0;  // qubits of deferred non-diagonal / diagonal ops
         std::vector<int> rest;
         for (int idx : remaining) {
-            const uint64_t qm = queue[idx].qmask();
-            if (qm & blocked) {
-                blocked |= qm;
+            const HostOp &h = queue[idx];
+            const uint64_t qm = h.qmask();
+            const bool diag = anywhere && is_diagonal(h);
+            const uint64_t block = (diag && s->fuse == 2) ? nd_block : (nd_block | d_block);
+            if (qm & block) {
+                (diag ? d_block : nd_block) |= qm;
                 rest.push_back(idx);
                 continue;
             }
-            if (popcount64(p.S | qm) <= b) {
-                p.S |= qm;
+            const uint64_t need = tile_need(h, anywhere);
+            if (popcount64(p.S | need) <= b) {
+                p.S |= need;
                 p.ops.push_back(idx);
             } else {
-                blocked |= qm;
+                nd_block |= qm;
                 rest.push_back(idx);
                 // program-order mode: the pass is a prefix of the queue
-                if (s->fuse == 1) blocked = ~uint64_t(0);
+                if (s->fuse == 1) nd_block = d_block = ~uint64_t(0);
             }
         }
         // fill the tile up to b qubits with the lowest unused qubits
@@ -236,19 +255,20 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     const int n = s->n;
     TilePass &pd = out.pd;
     std::memset(&pd, 0, sizeof pd);
-    int pos[64];
+    int pos[64];  // tile bit of a tile qubit; kOuterBit + outer-index bit otherwise
     int tb = 0, ob = 0;
     for (int q = 0; q < n; ++q) {
         if ((p.S >> q) & 1) {
             pd.tq[tb] = q;
             pos[q] = tb++;
         } else {
-            pd.oq[ob++] = q;
-            pos[q] = -1;
+            pd.oq[ob] = q;
+            pos[q] = kOuterBit + ob++;
         }
     }
     const int b = tb;
     out.b = b;
+    auto inner = [&](int bitpos) { return bitpos < kOuterBit; };
     pd.n_outer = ob;
     const unsigned lowmask = (1u << kLowQubits) - 1;
     const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - 4)) - 1);
@@ -257,7 +277,11 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     for (size_t i = 0; i < nops; ++i) {
         const HostOp &h = queue[p.ops[i]];
         unsigned qm = 0;
-        for (int j = 0; j < h.k; ++j) qm |= 1u << pos[h.q[j]];
+        bool all_inner = true;
+        for (int j = 0; j < h.k; ++j) {
+            if (inner(pos[h.q[j]])) qm |= 1u << pos[h.q[j]];
+            else all_inner = false;
+        }
         switch (h.kind) {
         case K_DENSE1:
         case K_MONO:
@@ -268,7 +292,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
             hard[i] = 1u << pos[h.q[h.k - 1]];
             break;
         case K_DIAG:
-            if (h.k <= 2) soft[i] = qm;
+            if (h.k <= 2 && all_inner) soft[i] = qm;
             break;
         default:
             break;
@@ -325,7 +349,11 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
         std::sort(t.sbit, t.sbit + h.k);
         const unsigned regmask = unpack_rb(lay[i]);
         unsigned qm = 0;
-        for (int j = 0; j < h.k; ++j) qm |= 1u << t.bit[j];
+        bool all_inner = true;
+        for (int j = 0; j < h.k; ++j) {
+            if (inner(t.bit[j])) qm |= 1u << t.bit[j];
+            else all_inner = false;
+        }
         const std::vector<double> *m = &h.mat;
         std::vector<double> tmp;
         switch (h.kind) {
@@ -350,12 +378,18 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
             t.kind = T_CTRL1;
             t.p = rb_slot(lay[i], t.bit[h.k - 1]);
             for (int j = 0; j < h.k - 1; ++j) {
-                t.cmask |= 1u << t.bit[j];
-                if (h.cval & (uint64_t(1) << h.q[j])) t.cval |= 1u << t.bit[j];
+                const bool one = (h.cval & (uint64_t(1) << h.q[j])) != 0;
+                if (inner(t.bit[j])) {
+                    t.cmask |= 1u << t.bit[j];
+                    if (one) t.cval |= 1u << t.bit[j];
+                } else {
+                    t.ocmask |= 1u << (t.bit[j] - kOuterBit);
+                    if (one) t.ocval |= 1u << (t.bit[j] - kOuterBit);
+                }
             }
             break;
         case K_DIAG:
-            if (h.k <= 2 && (qm & ~regmask) == 0) {
+            if (h.k <= 2 && all_inner && (qm & ~regmask) == 0) {
                 tmp = diag_to_dense(h.mat);
                 m = &tmp;
                 t.kind = h.k == 1 ? T_DENSE1 : T_DENSE2;
@@ -367,7 +401,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
                 t.p = -1;
                 t.q = -1;
                 for (int j = 0; j < h.k; ++j)
-                    if ((regmask >> t.bit[j]) & 1) {
+                    if (inner(t.bit[j]) && ((regmask >> t.bit[j]) & 1)) {
                         t.p = rb_slot(lay[i], t.bit[j]);
                         t.q = j;
                     }

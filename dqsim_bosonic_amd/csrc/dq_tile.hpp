@@ -259,7 +259,9 @@ __device__ __forceinline__ void from_lds(cplx<T> (&a)[16], const cplx<T> *lds, c
 template <typename T, int B>
 __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__ amps, TilePass pd,
                                                            const TileOp *__restrict__ ops,
-                                                           const cplx<T> *__restrict__ mats, uint64_t ntiles) {
+                                                           const cplx<T> *__restrict__ mats, uint64_t ntiles,
+                                                           double *__restrict__ partials) {
+    double red = 0.0;  // this thread's share of the epilogue reduction
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
     const unsigned t = threadIdx.x;
@@ -333,7 +335,8 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 dispatch_dense1<T>(a, op.p, op.variant, m);
                 break;
             case T_CTRL1:
-                dispatch_ctrl1<T>(a, op.p, op.variant, m, L, op.cmask, op.cval);
+                if ((unsigned(tile) & op.ocmask) == op.ocval)
+                    dispatch_ctrl1<T>(a, op.p, op.variant, m, L, op.cmask, op.cval);
                 break;
             case T_DENSE2:
                 dispatch_dense2<T>(a, op.p, op.q, op.variant, m);
@@ -342,8 +345,11 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 // e = thread part | register part; the thread part picks the
                 // candidate entries once per op (uniform values, no gathers).
                 int et = 0;
-                for (int b2 = 0; b2 < op.k; ++b2)
-                    if (b2 != op.q || op.p < 0) et |= int((L.tpart >> op.bit[b2]) & 1) << (op.k - 1 - b2);
+                for (int b2 = 0; b2 < op.k; ++b2) {
+                    const int bp = op.bit[b2];
+                    if (bp >= kOuterBit) et |= int((tile >> (bp - kOuterBit)) & 1) << (op.k - 1 - b2);
+                    else if (b2 != op.q || op.p < 0) et |= int((L.tpart >> bp) & 1) << (op.k - 1 - b2);
+                }
                 const cplx<T> m0 = m[0], m1 = m[1];
                 const cplx<T> m2 = op.k == 2 ? m[2] : m0, m3 = op.k == 2 ? m[3] : m1;
                 if (op.p < 0) {
@@ -363,16 +369,24 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 for (int j = 0; j < 16; ++j) {
                     const unsigned i = L.tpart | L.jp[j];
                     int e = 0;
-                    for (int b2 = 0; b2 < op.k; ++b2) e = (e << 1) | int((i >> op.bit[b2]) & 1);
+                    for (int b2 = 0; b2 < op.k; ++b2) {
+                        const int bp = op.bit[b2];
+                        const int v = bp >= kOuterBit ? int((tile >> (bp - kOuterBit)) & 1) : int((i >> bp) & 1);
+                        e = (e << 1) | v;
+                    }
                     a[j] = cmul(m[e], a[j]);
                 }
                 break;
             case T_COLLAPSE: {
                 const T nrm = T(op.norm);
+                const int bp = op.bit[0];
+                const bool outer = bp >= kOuterBit;
+                const int ov = outer ? int((tile >> (bp - kOuterBit)) & 1) : 0;
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     const unsigned i = L.tpart | L.jp[j];
-                    if (int((i >> op.bit[0]) & 1) == op.outcome) {
+                    const int v = outer ? ov : int((i >> (bp & 31)) & 1);
+                    if (v == op.outcome) {
                         a[j].re = a[j].re / nrm;
                         a[j].im = a[j].im / nrm;
                     } else {
@@ -416,6 +430,17 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             make_layout<T, B>(L, pd.store_rb, t);
             from_lds<T, B>(a, lds, L);
         }
+        if (pd.red_bit >= 0) {
+            const int rb2 = pd.red_bit;
+            const bool outer = rb2 >= kOuterBit;
+            const bool ov = outer && ((tile >> (rb2 - kOuterBit)) & 1);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const bool on = outer ? ov : (((L.tpart | L.jp[j]) >> (rb2 & 31)) & 1);
+                const double re = double(a[j].re), im = double(a[j].im);
+                if (on) red += re * re + im * im;
+            }
+        }
         const uint64_t sb = base + gs_t;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
@@ -426,6 +451,19 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             amps[off] = a[j];
         }
         // The next tile's first LDS write is preceded by a barrier.
+    }
+    if (pd.red_bit >= 0) {
+        // fixed-shape block sum: wave butterflies, then wave totals in order
+        for (int off = 32; off > 0; off >>= 1) red += __shfl_xor(red, off, 64);
+        __syncthreads();
+        double *scratch = reinterpret_cast<double *>(smem_raw);
+        if ((t & 63) == 0) scratch[t >> 6] = red;
+        __syncthreads();
+        if (t == 0) {
+            double r = 0.0;
+            for (int w = 0; w < (1 << (B - 4)) / 64; ++w) r += scratch[w];
+            partials[blockIdx.x] = r;
+        }
     }
 }
 

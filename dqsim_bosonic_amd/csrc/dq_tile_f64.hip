@@ -7,7 +7,7 @@ namespace {
 
 template <int B>
 hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass &pd, const TileOp *ops,
-                  const void *mats, uint64_t ntiles) {
+                  const void *mats, uint64_t ntiles, double *partials) {
     using T = double;
     const size_t shmem = sizeof(cplx<T>) << B;
     if (shmem > 65536) {
@@ -16,18 +16,18 @@ hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass 
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_tile_reg<T, B>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
-                       static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles);
+                       static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles, partials);
     return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t dq_launch_tile_reg_f64(int b, unsigned grid, hipStream_t stream, void *amps, const TilePass &pd,
-                                   const TileOp *ops, const void *mats, uint64_t ntiles) {
+                                   const TileOp *ops, const void *mats, uint64_t ntiles, double *partials) {
     switch (b) {
-    case 9: return launch<9>(grid, stream, amps, pd, ops, mats, ntiles);
-    case 12: return launch<12>(grid, stream, amps, pd, ops, mats, ntiles);
-    case 13: return launch<13>(grid, stream, amps, pd, ops, mats, ntiles);
+    case 9: return launch<9>(grid, stream, amps, pd, ops, mats, ntiles, partials);
+    case 12: return launch<12>(grid, stream, amps, pd, ops, mats, ntiles, partials);
+    case 13: return launch<13>(grid, stream, amps, pd, ops, mats, ntiles, partials);
     default: return hipErrorInvalidValue;
     }
 }

@@ -8,6 +8,8 @@
 #include "dq_tile.hpp"
 
 hipError_t dq_launch_tile_reg_f64(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
-                                  const dq::TileOp *ops, const void *mats, uint64_t ntiles);
+                                  const dq::TileOp *ops, const void *mats, uint64_t ntiles,
+                                  double *partials);
 hipError_t dq_launch_tile_reg_f32(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
-                                  const dq::TileOp *ops, const void *mats, uint64_t ntiles);
+                                  const dq::TileOp *ops, const void *mats, uint64_t ntiles,
+                                  double *partials);

@@ -36,7 +36,8 @@ enum TileKind : int {
     T_DIAG = 3,      // diagonal 2^k on any tile bits, elementwise
     T_COLLAPSE = 4,  // measurement collapse on tile bit bit[0], elementwise
     T_LDSOP = 5,     // any 2^k x 2^k matrix through LDS, one output per amplitude
-    T_DIAGT = 6      // diagonal, k <= 2, at most one register bit (slot p, gate-index bit q)
+    T_DIAGT = 6      // diagonal, k <= 2: outer bits, thread bits and at most one register bit
+                     // (slot p, gate-index bit q)
 };
 
 // Matrix shape of a register op, chosen on the host.  Every variant computes
@@ -56,11 +57,16 @@ struct TileOp {
     int mat;       // offset (complex entries) into the matrix pool
     int outcome;
     int variant;
-    int pad;
+    unsigned ocmask, ocval;  // T_CTRL1 controls on outer qubits (outer-index bits)
     double norm;
 };
 
+// A TileOp bit >= kOuterBit names outer-index bit (bit - kOuterBit) of the
+// tile: the qubit is not in the tile and its value is constant over the tile.
+constexpr int kOuterBit = 32;
+
 struct TilePass {
+    int red_bit;   // >= 0: sum |amp|^2 over the qubit at this tile/outer bit being 1 (per-block partials)
     int n_outer;
     int op_begin;
     int op_count;

@@ -231,3 +231,49 @@ def test_fusion_reduces_sweeps():
     st.synchronize()
     sweeps, launches, gates = st.stats()
     assert gates == 12 and sweeps == 1
+
+
+@pytest.mark.parametrize("n", [10, 16])
+def test_prob_one_with_pending_elementwise_ops_is_exact(n):
+    """p1 with queued diagonals/collapses pending (reduced without writing the
+    state) equals p1 reduced after those ops are applied, bit for bit."""
+    rng = np.random.default_rng(40 + n)
+    psi = random_state(n, rng)
+    results = []
+    for sync_first in (False, True):
+        st = capi.State(n)
+        st.set(psi)
+        st.apply_nq(orc.gate_matrix("rzz", [0.7]), [1, n - 1])
+        st.apply_1q(orc.gate_matrix("t"), 3)
+        st.collapse(2, 1, 0.9)
+        st.apply_nq(orc.gate_matrix("cp", [1.1]), [n - 2, 0])
+        if sync_first:
+            st.synchronize()
+        results.append([st.prob_one(q) for q in (0, n - 1, 5)])
+        st.close()
+    assert results[0] == results[1]
+
+
+@pytest.mark.parametrize("fuse", [0, 1, 2])
+def test_prob_one_fused_into_sweep(fuse):
+    """p1 summed by the last sweep of a flush agrees with the oracle."""
+    n = 17
+    rng = np.random.default_rng(55)
+    psi = random_state(n, rng)
+    prog = random_program(n, 40, rng)
+    for q in (0, 7, 16):
+        st = capi.State(n)
+        st.set_options(fuse, 0)
+        st.set(psi)
+        ref = psi.copy()
+        for item in prog:
+            if item[1] is not None:
+                u = orc.gate_matrix(item[0], item[1])
+                st.apply_1q(u, item[2][0])
+                orc.apply_one_qubit(ref, n, u, item[2][0])
+            else:
+                st.apply_nq(item[3], item[2])
+                orc.apply_n_qubit(ref, n, item[3], item[2])
+        p = st.prob_one(q)
+        assert abs(p - orc.prob_one(ref, n, q)) <= 1e-14
+        st.close()
