@@ -12,7 +12,7 @@ circuit through the dispatcher (gates, mid-circuit measurements with host
 ChaCha8 draws, resets, conditional); the circuit is decoded once, before
 timing.  value = circuit gates per second over the K timed steps.
 
-roofline: the dominant kernel is the LDS-tiled sweep (k_tile_pass); its
+roofline: the dominant kernel is the LDS-tiled sweep (k_tile_reg); its
 launches are bracketed by HIP events on the engine's stream.  Algorithmic
 bytes per launch = one read + one write of every amplitude = 2 * 16 * 2^n.
 
@@ -88,6 +88,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--fusion", type=int, default=2)
     ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -107,7 +108,7 @@ def main() -> int:
     text = circ.model_dump_json()
     log(f"[bench] workload random_blocks n={n} depth={args.depth} gates={gates}")
 
-    eng = _core.Engine(n, args.fusion, args.tile_bits)
+    eng = _core.Engine(n, args.fusion, args.tile_bits, args.low_bits)
     eng.load(text)
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -136,7 +137,7 @@ def main() -> int:
     achieved = alg_bytes / (avg_launch_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(n),
-                "kernel": "k_tile_pass<double,16>", "avg_launch_ms": round(avg_launch_ms, 4),
+                "kernel": "k_tile_reg<double,12>", "avg_launch_ms": round(avg_launch_ms, 4),
                 "launches_timed": int(timed), "sweeps_per_step": sweeps_per_step,
                 "kernel_time_frac_of_step": round(kernel_ms / 1e3 / elapsed, 4)}
 

@@ -18,7 +18,8 @@ ARCH = os.environ.get("DQ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off keeps every complex product/sum rounded as in the
-# reference (no FMA); packed fp32 ops are disabled for every HIP object.
+# reference (no FMA) in the exact objects; packed fp32 ops are disabled for
+# every HIP object.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -44,7 +45,10 @@ def core_path() -> str:
     return os.path.join(HERE, "_core" + suffix)
 
 
-ENGINE_SOURCES = ("dq_engine.hip", "dq_tile_f64.hip", "dq_tile_f32.hip")
+# (source, contraction): the exact kernels keep every product and sum rounded
+# separately; the contracted ones may fuse them into FMAs.
+ENGINE_SOURCES = (("dq_engine.hip", "off"), ("dq_tile_f64.hip", "off"), ("dq_tile_f64_fma.hip", "fast"),
+                  ("dq_tile_f32.hip", "fast"))
 
 
 def build_engine() -> str:
@@ -54,11 +58,11 @@ def build_engine() -> str:
     out = engine_path()
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
     jobs = []
-    for src in ENGINE_SOURCES:
+    for src, contract in ENGINE_SOURCES:
+        flags = [f if f != "-ffp-contract=off" else f"-ffp-contract={contract}" for f in HIP_FLAGS if f != "-shared"]
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        jobs.append(([HIPCC, *compile_flags, "-c", os.path.join(HERE, "csrc", src), "-o", obj], obj))
+        jobs.append(([HIPCC, *flags, "-c", os.path.join(HERE, "csrc", src), "-o", obj], obj))
     with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
         list(ex.map(lambda j: _run(j[0]), jobs))
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[o for _, o in jobs], "-o", out])

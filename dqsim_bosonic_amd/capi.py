@@ -51,7 +51,7 @@ def lib() -> ctypes.CDLL:
         L.dq_inner.argtypes = [vp, _dp, _dp, _dp]
         L.dq_copy_to_host.argtypes = [vp, _dp]
         L.dq_synchronize.argtypes = [vp]
-        L.dq_set_options.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.dq_set_options.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.dq_get_stats.argtypes = [vp, u64p, u64p, u64p]
         L.dq_set_timing.argtypes = [vp, ctypes.c_int]
@@ -142,8 +142,8 @@ class State:
     def synchronize(self) -> None:
         _check(lib().dq_synchronize(self.h))
 
-    def set_options(self, fuse: int = 1, tile_bits: int = 0) -> None:
-        _check(lib().dq_set_options(self.h, fuse, tile_bits))
+    def set_options(self, fuse: int = 1, tile_bits: int = 0, low_bits: int = 0) -> None:
+        _check(lib().dq_set_options(self.h, fuse, tile_bits, low_bits))
 
     def stats(self) -> tuple[int, int, int]:
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -608,11 +609,12 @@ int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *ma
     SweepTimer timer(s);
     const uint64_t ntiles = uint64_t(1) << rp.pd.n_outer;
     const uint64_t grid = std::min<uint64_t>(ntiles, 16384);
-    hipError_t e = std::is_same<T, double>::value
-                       ? dq_launch_tile_reg_f64(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles,
-                                                s->d_partials)
-                       : dq_launch_tile_reg_f32(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles,
-                                                s->d_partials);
+    // fusion level 1 keeps the reference's rounding: the exact kernel;
+    // level 2 already reorders commuting ops and uses the contracted kernel.
+    auto *fn = std::is_same<T, double>::value
+                   ? (s->fuse == 2 ? dq_launch_tile_reg_f64_fma : dq_launch_tile_reg_f64)
+                   : dq_launch_tile_reg_f32;
+    hipError_t e = fn(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles, s->d_partials);
     s->last_grid = int(grid);
     if (e == hipErrorInvalidValue) return fail(DQ_ERR_ARG, "unsupported tile size " + std::to_string(rp.b));
     HIP_TRY(e);
@@ -1124,15 +1126,17 @@ int dq_synchronize(dq_state *s) {
     return DQ_OK;
 }
 
-int dq_set_options(dq_state *s, int fuse, int tile_bits) {
+int dq_set_options(dq_state *s, int fuse, int tile_bits, int low_bits) {
     if (!s) return fail(DQ_ERR_ARG, "null state");
     if (int rc = flush(s)) return rc;
     const int max_bits = 13;
     if (tile_bits != 0 && (tile_bits < 6 || tile_bits > max_bits))
         return fail(DQ_ERR_ARG, "tile_bits out of range");
     if (fuse < 0 || fuse > 2) return fail(DQ_ERR_ARG, "fusion level must be 0, 1 or 2");
+    if (low_bits != 0 && (low_bits < 1 || low_bits > 8)) return fail(DQ_ERR_ARG, "low_bits out of range");
     s->fuse = fuse;
     if (tile_bits) s->tile_bits = tile_bits;
+    if (low_bits) s->low_bits = low_bits;
     return DQ_OK;
 }
 

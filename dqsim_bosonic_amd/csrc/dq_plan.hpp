@@ -193,7 +193,6 @@ inline std::vector<Pass> schedule(const PlanConfig *s, const std::vector<HostOp>
 
 // ---- register-resident tile passes (k_tile_reg) ----
 
-constexpr int kLowQubits = 5;  // qubits 0..4 stay thread bits at load/store (coalescing)
 
 inline int pack_rb(unsigned mask) {
     int packed = 0, m = 0;
@@ -270,7 +269,8 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     out.b = b;
     auto inner = [&](int bitpos) { return bitpos < kOuterBit; };
     pd.n_outer = ob;
-    const unsigned lowmask = (1u << kLowQubits) - 1;
+    // the low qubits stay thread bits at load/store (coalesced runs)
+    const unsigned lowmask = (1u << std::max(0, std::min(s->low_bits, b - 4))) - 1;
     const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - 4)) - 1);
     const size_t nops = p.ops.size();
     std::vector<unsigned> hard(nops, 0), soft(nops, 0);

@@ -241,22 +241,38 @@ __device__ __forceinline__ void dispatch_dense2(cplx<T> (&a)[16], int p, int q, 
     }
 }
 
+// The swizzle XORs nibbles together, so it is linear over XOR, and a tile
+// index is tpart | jp[j] with disjoint bits:
+//   swz(tpart | jp[j]) = swz(tpart) ^ swz(jp[j]),
+// one per-thread value per layout and 16 uniform ones.
 template <typename T, int B>
 __device__ __forceinline__ void to_lds(cplx<T> *lds, const cplx<T> (&a)[16], const Layout<T> &L) {
+    const unsigned st = swz<B>(L.tpart);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) lds[swz<B>(L.tpart | L.jp[j])] = a[j];
+    for (int j = 0; j < 16; ++j) lds[st ^ swz<B>(L.jp[j])] = a[j];
 }
 
 template <typename T, int B>
 __device__ __forceinline__ void from_lds(cplx<T> (&a)[16], const cplx<T> *lds, const Layout<T> &L) {
+    const unsigned st = swz<B>(L.tpart);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) a[j] = lds[swz<B>(L.tpart | L.jp[j])];
+    for (int j = 0; j < 16; ++j) a[j] = lds[st ^ swz<B>(L.jp[j])];
+}
+
+// Diagonal with one register bit (slot P): registers with that bit set take
+// d1, the others d0, chosen at compile time.
+template <typename T, int P>
+__device__ __forceinline__ void reg_diag_split(cplx<T> (&a)[16], cplx<T> d0, cplx<T> d1) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a[j] = cmul(((j >> P) & 1) ? d1 : d0, a[j]);
 }
 
 // One HBM sweep with register-resident amplitudes (see the file header).
 // The next tile's amplitudes are loaded into registers before the current
 // tile's ops run, so each tile's HBM latency overlaps the previous tile's work.
-template <typename T, int B>
+// FAST distinguishes the instantiation compiled with contraction (FMA) from
+// the exact one; the source is the same.
+template <typename T, int B, int FAST>
 __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__ amps, TilePass pd,
                                                            const TileOp *__restrict__ ops,
                                                            const cplx<T> *__restrict__ mats, uint64_t ntiles,
@@ -300,7 +316,12 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             nxt[j] = amps[off];
         }
     }
-    for (; tile < ntiles; tile += gridDim.x) {
+    // One tile: take the prefetched amplitudes, prefetch the next tile, run
+    // the pass's ops, store.  The first tile runs outside the loop so that
+    // the loop head is entered with the same memory operations pending from
+    // both sides (the next tile's loads, then this tile's stores); the wait
+    // for the loads then leaves the stores in flight.
+    auto process = [&](uint64_t tile) __attribute__((always_inline)) {
         const uint64_t base = tile_base(tile);
         cplx<T> a[16];
 #pragma unroll
@@ -359,8 +380,12 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 } else {
                     const int rw = 1 << (op.k - 1 - op.q);  // weight of the register bit in e
                     const cplx<T> d0 = pick4(et, m0, m1, m2, m3), d1 = pick4(et | rw, m0, m1, m2, m3);
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) a[j] = cmul(((j >> op.p) & 1) ? d1 : d0, a[j]);
+                    switch (op.p) {
+                    case 0: reg_diag_split<T, 0>(a, d0, d1); break;
+                    case 1: reg_diag_split<T, 1>(a, d0, d1); break;
+                    case 2: reg_diag_split<T, 2>(a, d0, d1); break;
+                    default: reg_diag_split<T, 3>(a, d0, d1); break;
+                    }
                 }
                 break;
             }
@@ -451,6 +476,10 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             amps[off] = a[j];
         }
         // The next tile's first LDS write is preceded by a barrier.
+    };
+    if (tile < ntiles) {
+        process(tile);
+        for (tile += gridDim.x; tile < ntiles; tile += gridDim.x) process(tile);
     }
     if (pd.red_bit >= 0) {
         // fixed-shape block sum: wave butterflies, then wave totals in order

@@ -1,4 +1,5 @@
 // dq_tile_f32.hip -- complex64 instantiations of the register-resident tile kernel.
+// Compiled with -ffp-contract=fast (the complex64 path makes no bit-exactness claim).
 #include "dq_tile_launch.hpp"
 
 using namespace dq;
@@ -11,11 +12,11 @@ hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass 
     using T = float;
     const size_t shmem = sizeof(cplx<T>) << B;
     if (shmem > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg<T, B>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg<T, B, 1>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_tile_reg<T, B>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
+    hipLaunchKernelGGL((k_tile_reg<T, B, 1>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
                        static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles, partials);
     return hipGetLastError();
 }

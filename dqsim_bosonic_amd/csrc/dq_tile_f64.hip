@@ -1,4 +1,5 @@
-// dq_tile_f64.hip -- complex128 instantiations of the register-resident tile kernel.
+// dq_tile_f64.hip -- complex128, exact arithmetic, instantiations of the register-resident tile kernel.
+// Compiled with -ffp-contract=off: every product and sum rounded separately, as the reference.
 #include "dq_tile_launch.hpp"
 
 using namespace dq;
@@ -11,11 +12,11 @@ hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass 
     using T = double;
     const size_t shmem = sizeof(cplx<T>) << B;
     if (shmem > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg<T, B>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg<T, B, 0>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_tile_reg<T, B>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
+    hipLaunchKernelGGL((k_tile_reg<T, B, 0>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
                        static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles, partials);
     return hipGetLastError();
 }

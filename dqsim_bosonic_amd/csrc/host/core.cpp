@@ -350,8 +350,9 @@ class Runner {
 
 class StatevectorSimulator {
    public:
-    StatevectorSimulator(std::optional<uint64_t> seed, bool profile, int fusion = 2, int tile_bits = 0)
-        : seed_(seed), profile_(profile), fusion_(fusion), tile_bits_(tile_bits) {}
+    StatevectorSimulator(std::optional<uint64_t> seed, bool profile, int fusion = 2, int tile_bits = 0,
+                         int low_bits = 0)
+        : seed_(seed), profile_(profile), fusion_(fusion), tile_bits_(tile_bits), low_bits_(low_bits) {}
 
     // simulator.rs:325-379
     SimulationResult simulate(py::object circuit) const {
@@ -366,7 +367,7 @@ class StatevectorSimulator {
         if (n64 > 40) throw py::value_error("circuit has " + std::to_string(n64) + " qubits; at most 40 are supported");
         const int n = int(n64);
         auto st = std::make_shared<DeviceState>(n);
-        check(dq_set_options(st->s, fusion_, tile_bits_), "options");
+        check(dq_set_options(st->s, fusion_, tile_bits_, low_bits_), "options");
         const uint64_t seed = seed_ ? *seed_ : ChaCha8Rng::entropy_u64();
         Runner run(st->s, n, seed, profile_);
         const auto t0 = Clock::now();
@@ -387,6 +388,7 @@ class StatevectorSimulator {
     bool profile_;
     int fusion_;
     int tile_bits_;
+    int low_bits_;
 };
 
 // Runs one decoded circuit repeatedly on a resident device state: the same
@@ -394,8 +396,8 @@ class StatevectorSimulator {
 // Used by the benchmark (one step = reset to |0...0> + the whole circuit).
 class Engine {
    public:
-    Engine(int n, int fusion, int tile_bits) : n_(n), st_(std::make_shared<DeviceState>(n)) {
-        check(dq_set_options(st_->s, fusion, tile_bits), "options");
+    Engine(int n, int fusion, int tile_bits, int low_bits) : n_(n), st_(std::make_shared<DeviceState>(n)) {
+        check(dq_set_options(st_->s, fusion, tile_bits, low_bits), "options");
     }
     void load(const std::string &text) {
         try {
@@ -498,8 +500,9 @@ PYBIND11_MODULE(_core, m) {
         .def("fidelity", &SimulationResult::fidelity, py::arg("other"));
 
     py::class_<StatevectorSimulator>(m, "StatevectorSimulator")
-        .def(py::init<std::optional<uint64_t>, bool, int, int>(), py::arg("seed") = py::none(),
-             py::arg("profile") = false, py::kw_only(), py::arg("fusion") = 2, py::arg("tile_bits") = 0)
+        .def(py::init<std::optional<uint64_t>, bool, int, int, int>(), py::arg("seed") = py::none(),
+             py::arg("profile") = false, py::kw_only(), py::arg("fusion") = 2, py::arg("tile_bits") = 0,
+             py::arg("low_bits") = 0)
         .def("simulate", &StatevectorSimulator::simulate, py::arg("circuit"));
 
     m.def(
@@ -523,7 +526,8 @@ PYBIND11_MODULE(_core, m) {
         py::arg("profile") = false);
 
     py::class_<Engine>(m, "Engine")
-        .def(py::init<int, int, int>(), py::arg("n"), py::arg("fusion") = 2, py::arg("tile_bits") = 0)
+        .def(py::init<int, int, int, int>(), py::arg("n"), py::arg("fusion") = 2, py::arg("tile_bits") = 0,
+             py::arg("low_bits") = 0)
         .def("load", &Engine::load)
         .def("step", &Engine::step, py::arg("seed"))
         .def("set_timing", &Engine::set_timing)

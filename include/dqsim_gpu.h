@@ -94,15 +94,17 @@ int dq_copy_to_host(dq_state *s, double *dst);
 /* Flushes queued gates and waits for the device. */
 int dq_synchronize(dq_state *s);
 
-/* Execution options (no reference counterpart; tuning only).
+/* Execution options (no reference counterpart; tuning only; 0 keeps a value).
  *   fuse = 0: every gate is its own HBM sweep;
  *   fuse = 1: gates are grouped into LDS-tiled sweeps in strict program
  *             order (bit-identical to the reference's arithmetic);
  *   fuse = 2: as 1, but a sweep may also take later gates on qubits disjoint
  *             from every gate it skipped (default; identical in exact
  *             arithmetic, last-bit rounding may differ).
- *   tile_bits = 0 keeps the current tile size (6..13 qubits per tile). */
-int dq_set_options(dq_state *s, int fuse, int tile_bits);
+ *   tile_bits: qubits per tile (6..13);
+ *   low_bits: qubits 0..low_bits-1 are in every tile, so each tile is made of
+ *             contiguous runs of 2^low_bits amplitudes (1..8). */
+int dq_set_options(dq_state *s, int fuse, int tile_bits, int low_bits);
 
 /* Sweep-kernel timing (no reference counterpart; measurement only).  With
  * timing on, every state-sweep kernel launch is bracketed by HIP events on
