@@ -19,6 +19,15 @@ bytes per launch = one read + one write of every amplitude = 2 * 16 * 2^n.
 cpu_baseline: the oracle's OpenMP restatement of the reference's Rayon
 kernels (src/engine.rs) on the host cores, on a bounded sample: the circuit's
 gates in program order from |0...0> until ~15 s have elapsed.
+
+adjoint (second half of the metric): gradient of E = <psi|H|psi> for the
+26-qubit, 40-layer ansatz (2080 trainable ry/rz parameters, rzz brickwork and
+crx on every 4th pair fixed; H = sum Z_i Z_{i+1} + 0.7 sum X_i), one parameter
+set drawn from the seed.  grads/s = full gradient vectors per second from the
+adjoint sweep; parameter shift is timed on a stated subset of parameters on the
+same GPU and extrapolated linearly to all of them; the subset also gives the
+adjoint's error against parameter shift (FP64) and the complex64 gradient's
+error against the FP64 one.
 """
 from __future__ import annotations
 
@@ -78,6 +87,42 @@ def cpu_baseline(circ, n: int, budget_s: float) -> dict:
                       f"(OpenMP restatement of src/engine.rs, {dt:.1f} s)"}
 
 
+def adjoint_leg(n: int, layers: int, seed: int, ps_subset: int) -> dict:
+    import numpy as np
+
+    from dqsim_bosonic_amd import adjoint as A
+
+    pc, H = A.hea_benchmark(n, layers, seed)
+    theta = np.random.default_rng(seed).uniform(0.0, 2 * np.pi, pc.num_params)
+    est = A.Estimator(pc, H, dtype="c128")
+    est.gradient(theta)  # warm-up
+    t0 = time.perf_counter()
+    e, g = est.gradient(theta)
+    t_adj = time.perf_counter() - t0
+    idx = np.random.default_rng(seed + 1).choice(pc.num_params, size=ps_subset, replace=False)
+    t0 = time.perf_counter()
+    ps = est.parameter_shift(theta, indices=idx)
+    t_ps_sub = time.perf_counter() - t0
+    t_ps_full = t_ps_sub * pc.num_params / ps_subset
+    err64 = float(np.max(np.abs(g[idx] - ps)))
+    est32 = A.Estimator(pc, H, dtype="c64")
+    est32.gradient(theta)
+    t0 = time.perf_counter()
+    e32, g32 = est32.gradient(theta)
+    t_adj32 = time.perf_counter() - t0
+    rel32 = float(np.linalg.norm(g32 - g) / np.linalg.norm(g))
+    return {"workload": f"hea {n}q {layers} layers, {pc.num_params} params, H = ZZ chain + 0.7 X sum",
+            "energy": e, "grads_per_s": round(1.0 / t_adj, 4), "adjoint_s": round(t_adj, 3),
+            "param_shift_grads_per_s": round(1.0 / t_ps_full, 6),
+            "param_shift_s_extrapolated": round(t_ps_full, 1),
+            "param_shift_sample": f"{ps_subset} of {pc.num_params} parameters ({t_ps_sub:.2f} s), "
+                                  "extrapolated linearly",
+            "speedup_vs_param_shift": round(t_ps_full / t_adj, 1),
+            "max_abs_err_vs_param_shift_f64": err64,
+            "c64_grads_per_s": round(1.0 / t_adj32, 4),
+            "c64_rel_err_vs_f64": rel32, "c64_energy_err": abs(e32 - e)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +136,10 @@ def main() -> int:
     ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-qubits", type=int, default=26)
+    ap.add_argument("--adjoint-layers", type=int, default=40)
+    ap.add_argument("--ps-subset", type=int, default=8)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -141,6 +190,13 @@ def main() -> int:
                 "launches_timed": int(timed), "sweeps_per_step": sweeps_per_step,
                 "kernel_time_frac_of_step": round(kernel_ms / 1e3 / elapsed, 4)}
 
+    adj = None
+    if not args.no_adjoint:
+        log("[bench] adjoint gradient ...")
+        del eng  # free the 30-qubit state first
+        adj = adjoint_leg(args.adjoint_qubits, args.adjoint_layers, args.seed, args.ps_subset)
+        log(f"[bench] adjoint {adj}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -157,6 +213,7 @@ def main() -> int:
                    "fusion": args.fusion, "measurement_record": records[0]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -21,7 +21,8 @@ ERR_ARG, ERR_OOM, ERR_DEVICE = -1, -2, -3
 EXPORTS = (
     "dq_state_create", "dq_state_destroy", "dq_state_num_qubits", "dq_state_set", "dq_state_reset",
     "dq_apply_1q", "dq_apply_nq", "dq_prob_one", "dq_collapse", "dq_marginal_probs", "dq_inner",
-    "dq_copy_to_host", "dq_synchronize", "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
+    "dq_copy_to_host", "dq_synchronize", "dq_state_copy", "dq_apply_pauli_sum", "dq_pauli_inner",
+    "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
     "dq_last_error",
 )
 
@@ -54,6 +55,9 @@ def lib() -> ctypes.CDLL:
         L.dq_set_options.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.dq_get_stats.argtypes = [vp, u64p, u64p, u64p]
+        L.dq_state_copy.argtypes = [vp, vp]
+        L.dq_apply_pauli_sum.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp]
+        L.dq_pauli_inner.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp, _dp, _dp]
         L.dq_set_timing.argtypes = [vp, ctypes.c_int]
         L.dq_get_timing.argtypes = [vp, _dp, u64p]
         L.dq_last_error.restype = ctypes.c_char_p
@@ -138,6 +142,28 @@ class State:
         out = np.empty(1 << self.n, dtype=np.complex128)
         _check(lib().dq_copy_to_host(self.h, _d(out.view(np.float64))))
         return out
+
+    def copy_from(self, other: "State") -> None:
+        _check(lib().dq_state_copy(self.h, other.h))
+
+    @staticmethod
+    def _terms(terms):
+        """terms: iterable of (coeff, xmask, zmask)."""
+        terms = list(terms)
+        x = (ctypes.c_uint64 * max(1, len(terms)))(*[int(t[1]) for t in terms])
+        z = (ctypes.c_uint64 * max(1, len(terms)))(*[int(t[2]) for t in terms])
+        c = np.array([[complex(t[0]).real, complex(t[0]).imag] for t in terms] or [[0.0, 0.0]], dtype=np.float64)
+        return len(terms), x, z, c
+
+    def apply_pauli_sum(self, src: "State", terms) -> None:
+        k, x, z, c = self._terms(terms)
+        _check(lib().dq_apply_pauli_sum(self.h, src.h, k, x, z, _d(c)))
+
+    def pauli_inner(self, ket: "State", terms) -> complex:
+        k, x, z, c = self._terms(terms)
+        re, im = ctypes.c_double(), ctypes.c_double()
+        _check(lib().dq_pauli_inner(self.h, ket.h, k, x, z, _d(c), ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
 
     def synchronize(self) -> None:
         _check(lib().dq_synchronize(self.h))

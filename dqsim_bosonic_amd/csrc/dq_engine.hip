@@ -280,6 +280,76 @@ __global__ __launch_bounds__(kThreads) void k_inner(const cplx<T> *__restrict__ 
     }
 }
 
+// ---- Pauli sums (observables and adjoint gradients) ----
+//
+// A Pauli string on n qubits is (x, z): X on the bits of x, Z on the bits of
+// z, Y where both are set, with Y = i X Z.  P|y> = i^{#Y} (-1)^{popcount(z&y)}
+// |y ^ x>, so (P psi)[r] = i^{#Y} (-1)^{popcount(z & (r ^ x))} psi[r ^ x].
+
+struct PauliTerm {
+    uint64_t x, z;
+    double cre, cim;  // coefficient times i^{#Y}
+};
+
+// dst[r] = sum_k c_k (P_k src)[r]; terms in the order given.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_apply_pauli_sum(cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ src,
+                                                              uint64_t count, const PauliTerm *__restrict__ terms,
+                                                              int nterms) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < count; r += stride) {
+        double ar = 0.0, ai = 0.0;
+        for (int k = 0; k < nterms; ++k) {
+            const PauliTerm t = terms[k];
+            const uint64_t y = r ^ t.x;
+            const cplx<T> v = src[y];
+            const double sgn = (__popcll(t.z & y) & 1) ? -1.0 : 1.0;
+            const double vr = sgn * double(v.re), vi = sgn * double(v.im);
+            ar += t.cre * vr - t.cim * vi;
+            ai += t.cre * vi + t.cim * vr;
+        }
+        dst[r] = cplx<T>{T(ar), T(ai)};
+    }
+}
+
+// partials[block] = per-block sum over r of conj(bra[r]) * sum_k c_k (P_k ket)[r].
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_pauli_inner(const cplx<T> *__restrict__ bra, const cplx<T> *__restrict__ ket,
+                                                          uint64_t count, const PauliTerm *__restrict__ terms, int nterms,
+                                                          double *__restrict__ partials) {
+    __shared__ double scratch[kThreads / 64];
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    double sr = 0.0, si = 0.0;
+    for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < count; r += stride) {
+        const double br = double(bra[r].re), bi = -double(bra[r].im);
+        double ar = 0.0, ai = 0.0;
+        for (int k = 0; k < nterms; ++k) {
+            const PauliTerm t = terms[k];
+            const uint64_t y = r ^ t.x;
+            const cplx<T> v = ket[y];
+            const double sgn = (__popcll(t.z & y) & 1) ? -1.0 : 1.0;
+            const double vr = sgn * double(v.re), vi = sgn * double(v.im);
+            ar += t.cre * vr - t.cim * vi;
+            ai += t.cre * vi + t.cim * vr;
+        }
+        sr += br * ar - bi * ai;
+        si += br * ai + bi * ar;
+    }
+    const double r0 = block_sum(sr, scratch);
+    const double r1 = block_sum(si, scratch);
+    if (threadIdx.x == 0) {
+        partials[2 * size_t(blockIdx.x)] = r0;
+        partials[2 * size_t(blockIdx.x) + 1] = r1;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_copy_state(cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ src,
+                                                         uint64_t count) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = src[i];
+}
+
 // Full distribution over all n qubits in a given order: out[j(i)] = |psi_i|^2.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_probs_perm(const cplx<T> *__restrict__ amps, int n,
@@ -1123,6 +1193,128 @@ int dq_synchronize(dq_state *s) {
     if (!s) return fail(DQ_ERR_ARG, "null state");
     if (int rc = flush(s)) return rc;
     HIP_TRY(hipStreamSynchronize(s->stream));
+    return DQ_OK;
+}
+
+namespace {
+
+// Builds the device term list; Y phases folded into the coefficients.
+int pauli_terms(const dq_state *s, int n_terms, const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                std::vector<PauliTerm> &out) {
+    if (n_terms < 0 || n_terms > 4096) return fail(DQ_ERR_ARG, "number of Pauli terms must be in [0, 4096]");
+    if (n_terms > 0 && (!xmask || !zmask || !coeff)) return fail(DQ_ERR_ARG, "null Pauli term arrays");
+    const uint64_t full = s->n >= 64 ? ~uint64_t(0) : ((uint64_t(1) << s->n) - 1);
+    out.resize(size_t(n_terms));
+    for (int k = 0; k < n_terms; ++k) {
+        if ((xmask[k] | zmask[k]) & ~full) return fail(DQ_ERR_ARG, "Pauli term acts on a qubit outside the state");
+        const int ny = __builtin_popcountll(xmask[k] & zmask[k]) & 3;
+        double re = coeff[2 * k], im = coeff[2 * k + 1];
+        for (int j = 0; j < ny; ++j) {  // multiply by i
+            const double t = re;
+            re = -im;
+            im = t;
+        }
+        out[k] = PauliTerm{xmask[k], zmask[k], re, im};
+    }
+    return DQ_OK;
+}
+
+int upload_terms(dq_state *s, const std::vector<PauliTerm> &terms, PauliTerm **d_terms) {
+    const size_t bytes = std::max<size_t>(terms.size(), 1) * sizeof(PauliTerm);
+    HIP_TRY(hipEventSynchronize(s->upload_done));
+    if (int rc = ensure_host_buffer(s, bytes)) return rc;
+    if (int rc = ensure_device_buffer(s, bytes)) return rc;
+    std::memcpy(s->h_upload, terms.data(), terms.size() * sizeof(PauliTerm));
+    HIP_TRY(hipMemcpyAsync(s->d_upload, s->h_upload, bytes, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipEventRecord(s->upload_done, s->stream));
+    *d_terms = static_cast<PauliTerm *>(s->d_upload);
+    return DQ_OK;
+}
+
+// Makes `b`'s pending work visible on `a`'s stream (both flushed first).
+int join(dq_state *a, dq_state *b) {
+    if (int rc = flush(a)) return rc;
+    if (a == b) return DQ_OK;
+    if (int rc = flush(b)) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return DQ_OK;
+}
+
+}  // namespace
+
+int dq_state_copy(dq_state *dst, dq_state *src) {
+    if (!dst || !src) return fail(DQ_ERR_ARG, "null state");
+    if (dst->n != src->n || dst->dtype != src->dtype) return fail(DQ_ERR_ARG, "states differ in size or dtype");
+    if (dst == src) return flush(dst);
+    if (int rc = join(dst, src)) return rc;
+    dst->queue.clear();
+    const uint64_t count = amp_count(dst);
+    if (dst->dtype == DQ_C128)
+        hipLaunchKernelGGL(k_copy_state<double>, dim3(grid_for(count)), dim3(kThreads), 0, dst->stream,
+                           static_cast<cplx<double> *>(dst->amps), static_cast<const cplx<double> *>(src->amps), count);
+    else
+        hipLaunchKernelGGL(k_copy_state<float>, dim3(grid_for(count)), dim3(kThreads), 0, dst->stream,
+                           static_cast<cplx<float> *>(dst->amps), static_cast<const cplx<float> *>(src->amps), count);
+    HIP_TRY(hipGetLastError());
+    dst->launches++;
+    // src may be modified on its own stream right after this call returns
+    HIP_TRY(hipStreamSynchronize(dst->stream));
+    return DQ_OK;
+}
+
+int dq_apply_pauli_sum(dq_state *dst, dq_state *src, int n_terms, const uint64_t *xmask, const uint64_t *zmask,
+                       const double *coeff) {
+    if (!dst || !src) return fail(DQ_ERR_ARG, "null state");
+    if (dst == src) return fail(DQ_ERR_ARG, "dq_apply_pauli_sum needs distinct source and destination states");
+    if (dst->n != src->n || dst->dtype != src->dtype) return fail(DQ_ERR_ARG, "states differ in size or dtype");
+    std::vector<PauliTerm> terms;
+    if (int rc = pauli_terms(src, n_terms, xmask, zmask, coeff, terms)) return rc;
+    if (int rc = join(dst, src)) return rc;
+    dst->queue.clear();
+    PauliTerm *d_terms;
+    if (int rc = upload_terms(dst, terms, &d_terms)) return rc;
+    const uint64_t count = amp_count(dst);
+    if (dst->dtype == DQ_C128)
+        hipLaunchKernelGGL(k_apply_pauli_sum<double>, dim3(grid_for(count)), dim3(kThreads), 0, dst->stream,
+                           static_cast<cplx<double> *>(dst->amps), static_cast<const cplx<double> *>(src->amps), count,
+                           d_terms, n_terms);
+    else
+        hipLaunchKernelGGL(k_apply_pauli_sum<float>, dim3(grid_for(count)), dim3(kThreads), 0, dst->stream,
+                           static_cast<cplx<float> *>(dst->amps), static_cast<const cplx<float> *>(src->amps), count,
+                           d_terms, n_terms);
+    HIP_TRY(hipGetLastError());
+    dst->launches++;
+    // src may be modified on its own stream right after this call returns
+    HIP_TRY(hipStreamSynchronize(dst->stream));
+    return DQ_OK;
+}
+
+int dq_pauli_inner(dq_state *bra, dq_state *ket, int n_terms, const uint64_t *xmask, const uint64_t *zmask,
+                   const double *coeff, double *re, double *im) {
+    if (!bra || !ket || !re || !im) return fail(DQ_ERR_ARG, "null argument");
+    if (bra->n != ket->n || bra->dtype != ket->dtype) return fail(DQ_ERR_ARG, "states differ in size or dtype");
+    std::vector<PauliTerm> terms;
+    if (int rc = pauli_terms(ket, n_terms, xmask, zmask, coeff, terms)) return rc;
+    if (int rc = join(ket, bra)) return rc;
+    PauliTerm *d_terms;
+    if (int rc = upload_terms(ket, terms, &d_terms)) return rc;
+    const uint64_t count = amp_count(ket);
+    const int blocks = std::min<int>(kReduceBlocks, grid_for(count));
+    if (ket->dtype == DQ_C128)
+        hipLaunchKernelGGL(k_pauli_inner<double>, dim3(blocks), dim3(kThreads), 0, ket->stream,
+                           static_cast<const cplx<double> *>(bra->amps), static_cast<const cplx<double> *>(ket->amps),
+                           count, d_terms, n_terms, ket->d_partials);
+    else
+        hipLaunchKernelGGL(k_pauli_inner<float>, dim3(blocks), dim3(kThreads), 0, ket->stream,
+                           static_cast<const cplx<float> *>(bra->amps), static_cast<const cplx<float> *>(ket->amps),
+                           count, d_terms, n_terms, ket->d_partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kThreads), 0, ket->stream, ket->d_partials, blocks, 2,
+                       ket->d_result);
+    HIP_TRY(hipGetLastError());
+    ket->launches += 2;
+    if (int rc = fetch_result(ket, 2)) return rc;
+    *re = ket->h_result[0];
+    *im = ket->h_result[1];
     return DQ_OK;
 }
 

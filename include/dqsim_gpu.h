@@ -91,6 +91,22 @@ int dq_inner(dq_state *s, const double *other, double *re, double *im);
  * (src/monolithic/statevector/simulator.rs:122-125). */
 int dq_copy_to_host(dq_state *s, double *dst);
 
+/* ---- observables (no reference counterpart: the adjoint-gradient form) ----
+ * A Pauli string is (xmask, zmask): X on xmask's bits, Z on zmask's bits, Y
+ * where both are set (Y = i X Z).  coeff holds n_terms (re, im) pairs. */
+
+/* dst = src (amplitudes; both states must have the same size and dtype). */
+int dq_state_copy(dq_state *dst, dq_state *src);
+
+/* dst = sum_k coeff_k P_k src  (dst and src distinct). */
+int dq_apply_pauli_sum(dq_state *dst, dq_state *src, int n_terms, const uint64_t *xmask,
+                       const uint64_t *zmask, const double *coeff);
+
+/* (re, im) = sum_k coeff_k <bra| P_k |ket>, FP64 accumulation; synchronous.
+ * With one identity term (0, 0, 1) this is <bra|ket>. */
+int dq_pauli_inner(dq_state *bra, dq_state *ket, int n_terms, const uint64_t *xmask,
+                   const uint64_t *zmask, const double *coeff, double *re, double *im);
+
 /* Flushes queued gates and waits for the device. */
 int dq_synchronize(dq_state *s);
 
