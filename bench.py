@@ -28,6 +28,13 @@ adjoint sweep; parameter shift is timed on a stated subset of parameters on the
 same GPU and extrapolated linearly to all of them; the subset also gives the
 adjoint's error against parameter shift (FP64) and the complex64 gradient's
 error against the FP64 one.
+
+precision (complex64 form): the same circuit family at 32 qubits, depth 60,
+blocks of 2 x 16 qubits, run as complex64 with fusion level 3 (runs of gates
+on <= 5 qubits multiplied into 32x32 blocks in FP64 and applied on FP32 MFMA
+inside the LDS-tiled sweep) and as complex128 (fusion level 2) on the same GPU
+(32 GiB + 64 GiB).  Reported: gates/s of each, and on device max |d amp|,
+1 - |<psi64|psi32>|^2 and the total variation distance of the probabilities.
 """
 from __future__ import annotations
 
@@ -123,6 +130,55 @@ def adjoint_leg(n: int, layers: int, seed: int, ps_subset: int) -> dict:
             "c64_rel_err_vs_f64": rel32, "c64_energy_err": abs(e32 - e)}
 
 
+def precision_leg(n: int, depth: int, seed: int, steps: int) -> dict:
+    from dqsim_bosonic_amd import _core
+    from dqsim_bosonic_amd import workloads as W
+
+    circ = W.random_blocks(n, depth, seed=seed)
+    gates = W.gate_count(circ)
+    text = circ.model_dump_json()
+    out = {"workload": f"random_blocks {n}q depth {depth} ({gates} gates), blocks 2x{n // 2}"}
+    e32 = _core.Engine(n, 3, 0, 0, 1)
+    e32.load(text)
+    rec32 = e32.step(seed)  # warm-up
+    e32.set_timing(True)
+    f0 = e32.fused_stats()
+    s0 = e32.stats()[0]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rec32 = e32.step(seed)
+    dt32 = (time.perf_counter() - t0) / steps
+    ms32, cnt32 = e32.timing()
+    f1 = e32.fused_stats()
+    s1 = e32.stats()[0]
+    e32.set_timing(False)
+    log(f"[bench] precision c64: {dt32:.3f} s/step")
+    e64 = _core.Engine(n, 2, 0, 0, 0)
+    e64.load(text)
+    t0 = time.perf_counter()
+    rec64 = e64.step(seed)
+    dt64 = time.perf_counter() - t0
+    log(f"[bench] precision c128: {dt64:.3f} s/step")
+    cmp = e32.compare(e64)
+    avg32 = ms32 / max(cnt32, 1)
+    blocks = (f1[0] - f0[0]) / steps
+    alg_bytes = 2.0 * 8.0 * float(1 << n)
+    # complex 32x32 GEMM per 32-amplitude group: 32*32 complex MACs = 8192 flops
+    gemm_flops_per_sweep = blocks / max((s1 - s0) / steps, 1) * 8192.0 * float(1 << (n - 5))
+    out.update({
+        "c64_gates_per_s": round(gates / dt32, 2), "c128_gates_per_s": round(gates / dt64, 2),
+        "c64_s_per_step": round(dt32, 3), "c128_s_per_step": round(dt64, 3),
+        "measurement_records_equal": rec32 == rec64,
+        "max_abs_err": cmp["max_abs"], "fidelity_deficit": 1.0 - cmp["fidelity"], "tvd": cmp["tvd"],
+        "fused_blocks_per_step": blocks, "fused_gates_per_step": (f1[1] - f0[1]) / steps,
+        "sweeps_per_step": (s1 - s0) / steps,
+        "c64_avg_sweep_ms": round(avg32, 4),
+        "c64_sweep_hbm_frac": round(alg_bytes / (avg32 / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "c64_sweep_mfma_frac": round(gemm_flops_per_sweep / (avg32 / 1e3) / 157.3e12, 4),
+    })
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +196,9 @@ def main() -> int:
     ap.add_argument("--adjoint-qubits", type=int, default=26)
     ap.add_argument("--adjoint-layers", type=int, default=40)
     ap.add_argument("--ps-subset", type=int, default=8)
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-qubits", type=int, default=32)
+    ap.add_argument("--precision-depth", type=int, default=60)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -197,6 +256,12 @@ def main() -> int:
         adj = adjoint_leg(args.adjoint_qubits, args.adjoint_layers, args.seed, args.ps_subset)
         log(f"[bench] adjoint {adj}")
 
+    prec = None
+    if not args.no_precision:
+        log("[bench] precision (complex64 + MFMA blocks vs complex128) ...")
+        prec = precision_leg(args.precision_qubits, args.precision_depth, args.seed, 1)
+        log(f"[bench] precision {prec}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -214,6 +279,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "precision": prec,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -22,7 +22,9 @@ EXPORTS = (
     "dq_state_create", "dq_state_destroy", "dq_state_num_qubits", "dq_state_set", "dq_state_reset",
     "dq_apply_1q", "dq_apply_nq", "dq_prob_one", "dq_collapse", "dq_marginal_probs", "dq_inner",
     "dq_copy_to_host", "dq_synchronize", "dq_state_copy", "dq_apply_pauli_sum", "dq_pauli_inner",
+    "dq_compare_states",
     "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
+    "dq_get_fused_stats",
     "dq_last_error",
 )
 
@@ -55,7 +57,9 @@ def lib() -> ctypes.CDLL:
         L.dq_set_options.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.dq_get_stats.argtypes = [vp, u64p, u64p, u64p]
+        L.dq_get_fused_stats.argtypes = [vp, u64p, u64p]
         L.dq_state_copy.argtypes = [vp, vp]
+        L.dq_compare_states.argtypes = [vp, vp, _dp, _dp, _dp, _dp]
         L.dq_apply_pauli_sum.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp]
         L.dq_pauli_inner.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp, _dp, _dp]
         L.dq_set_timing.argtypes = [vp, ctypes.c_int]
@@ -165,6 +169,13 @@ class State:
         _check(lib().dq_pauli_inner(self.h, ket.h, k, x, z, _d(c), ctypes.byref(re), ctypes.byref(im)))
         return complex(re.value, im.value)
 
+    def compare(self, other: "State") -> dict:
+        m, r, i, t = (ctypes.c_double() for _ in range(4))
+        _check(lib().dq_compare_states(self.h, other.h, ctypes.byref(m), ctypes.byref(r), ctypes.byref(i),
+                                       ctypes.byref(t)))
+        inner = complex(r.value, i.value)
+        return {"max_abs": m.value, "inner": inner, "fidelity": abs(inner) ** 2, "tvd": t.value}
+
     def synchronize(self) -> None:
         _check(lib().dq_synchronize(self.h))
 
@@ -175,3 +186,9 @@ class State:
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().dq_get_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def fused_stats(self) -> tuple[int, int]:
+        """(5-qubit MFMA blocks applied, gates they replaced) under fusion level 3."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().dq_get_fused_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

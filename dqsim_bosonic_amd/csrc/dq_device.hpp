@@ -67,53 +67,85 @@ __device__ __forceinline__ cplx<T> pick(const cplx<T> (&v)[DIM], int idx) {
 }
 
 // Monomial gate: out[r] = ph[r] * in[perm[r]] on every group of 2^K.
+// Up to 3 qubits the offsets, permutation and phases stay in registers for
+// the whole sweep; for 4-5 qubits they are re-read per row (a hoisted copy of
+// 32 rows would spill to scratch memory).
 template <typename T, typename I, int K>
 __device__ __forceinline__ void mono_k(cplx<T> *__restrict__ a, int nbits, const DevOp &op,
                                        const cplx<T> *__restrict__ mats,
                                        const int *__restrict__ perms, I tid, I nthr) {
     constexpr int DIM = 1 << K;
-    I off[DIM];
-#pragma unroll
-    for (int j = 0; j < DIM; ++j) off[j] = group_offset<I>(j, op.bit, K);
-    int pm[DIM];
-    cplx<T> ph[DIM];
-#pragma unroll
-    for (int r = 0; r < DIM; ++r) {
-        pm[r] = perms[op.aux + r];
-        ph[r] = mats[op.mat + r];
-    }
     const I ngroups = I(1) << (nbits - K);
-    for (I g = tid; g < ngroups; g += nthr) {
-        const I base = insert_zeros<I>(g, op.sbit, K);
-        cplx<T> v[DIM];
+    if constexpr (K <= 3) {
+        I off[DIM];
+        int pm[DIM];
+        cplx<T> ph[DIM];
 #pragma unroll
-        for (int j = 0; j < DIM; ++j) v[j] = a[base | off[j]];
+        for (int r = 0; r < DIM; ++r) {
+            off[r] = group_offset<I>(r, op.bit, K);
+            pm[r] = perms[op.aux + r];
+            ph[r] = mats[op.mat + r];
+        }
+        for (I g = tid; g < ngroups; g += nthr) {
+            const I base = insert_zeros<I>(g, op.sbit, K);
+            cplx<T> v[DIM];
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) a[base | off[r]] = cmul(ph[r], pick<T, DIM>(v, pm[r]));
+            for (int j = 0; j < DIM; ++j) v[j] = a[base | off[j]];
+#pragma unroll
+            for (int r = 0; r < DIM; ++r) a[base | off[r]] = cmul(ph[r], pick<T, DIM>(v, pm[r]));
+        }
+    } else {
+        for (I g = tid; g < ngroups; g += nthr) {
+            const I base = insert_zeros<I>(g, op.sbit, K);
+            cplx<T> v[DIM];
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) v[j] = a[base | group_offset<I>(j, op.bit, K)];
+#pragma unroll 1
+            for (int r = 0; r < DIM; ++r)
+                a[base | group_offset<I>(r, op.bit, K)] = cmul(mats[op.mat + r], pick<T, DIM>(v, perms[op.aux + r]));
+        }
     }
 }
 
 // Dense gate, the reference's row loop: acc = 0; acc += U[r][c] * v[c].
+// Rows are fully unrolled up to 3 qubits; for 4-5 qubits the row loop stays
+// a loop so the 2^2K matrix entries are not hoisted out of the sweep.
 template <typename T, typename I, int K>
 __device__ __forceinline__ void dense_k(cplx<T> *__restrict__ a, int nbits, const DevOp &op,
                                         const cplx<T> *__restrict__ mats, I tid, I nthr) {
     constexpr int DIM = 1 << K;
-    I off[DIM];
-#pragma unroll
-    for (int j = 0; j < DIM; ++j) off[j] = group_offset<I>(j, op.bit, K);
     const cplx<T> *__restrict__ U = mats + op.mat;
     const I ngroups = I(1) << (nbits - K);
-    for (I g = tid; g < ngroups; g += nthr) {
-        const I base = insert_zeros<I>(g, op.sbit, K);
-        cplx<T> v[DIM];
+    if constexpr (K <= 3) {
+        I off[DIM];
 #pragma unroll
-        for (int j = 0; j < DIM; ++j) v[j] = a[base | off[j]];
+        for (int j = 0; j < DIM; ++j) off[j] = group_offset<I>(j, op.bit, K);
+        for (I g = tid; g < ngroups; g += nthr) {
+            const I base = insert_zeros<I>(g, op.sbit, K);
+            cplx<T> v[DIM];
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-            cplx<T> acc{T(0), T(0)};
+            for (int j = 0; j < DIM; ++j) v[j] = a[base | off[j]];
 #pragma unroll
-            for (int c = 0; c < DIM; ++c) acc = cadd(acc, cmul(U[r * DIM + c], v[c]));
-            a[base | off[r]] = acc;
+            for (int r = 0; r < DIM; ++r) {
+                cplx<T> acc{T(0), T(0)};
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) acc = cadd(acc, cmul(U[r * DIM + c], v[c]));
+                a[base | off[r]] = acc;
+            }
+        }
+    } else {
+        for (I g = tid; g < ngroups; g += nthr) {
+            const I base = insert_zeros<I>(g, op.sbit, K);
+            cplx<T> v[DIM];
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) v[j] = a[base | group_offset<I>(j, op.bit, K)];
+#pragma unroll 1
+            for (int r = 0; r < DIM; ++r) {
+                cplx<T> acc{T(0), T(0)};
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) acc = cadd(acc, cmul(U[r * DIM + c], v[c]));
+                a[base | group_offset<I>(r, op.bit, K)] = acc;
+            }
         }
     }
 }
@@ -169,12 +201,12 @@ __device__ __forceinline__ void apply_op(cplx<T> *__restrict__ a, int nbits, con
     case K_COLLAPSE: {
         const I total = I(1) << nbits;
         const int p = op.bit[0];
-        const T nrm = T(op.norm);
+        const double nrm = op.norm;  // renormalise in FP64 (exact division for T = double)
         for (I i = tid; i < total; i += nthr) {
             if (int((i >> p) & 1) == op.outcome) {
                 cplx<T> v = a[i];
-                v.re = v.re / nrm;
-                v.im = v.im / nrm;
+                v.re = T(double(v.re) / nrm);
+                v.im = T(double(v.im) / nrm);
                 a[i] = v;
             } else {
                 a[i] = cplx<T>{T(0), T(0)};

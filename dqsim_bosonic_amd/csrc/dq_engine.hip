@@ -221,9 +221,8 @@ __global__ __launch_bounds__(kThreads) void k_prob_one_virtual(const cplx<T> *__
                 a = cmul(mats[op.mat + gather_bits<uint64_t>(i, op.bit, op.k)], a);
             } else {  // K_COLLAPSE
                 if (int((i >> op.bit[0]) & 1) == op.outcome) {
-                    const T nrm = T(op.norm);
-                    a.re = a.re / nrm;
-                    a.im = a.im / nrm;
+                    a.re = T(double(a.re) / op.norm);
+                    a.im = T(double(a.im) / op.norm);
                 } else {
                     a = cplx<T>{T(0), T(0)};
                 }
@@ -350,6 +349,39 @@ __global__ __launch_bounds__(kThreads) void k_copy_state(cplx<T> *__restrict__ d
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = src[i];
 }
 
+// Differences between two states of the same size (any dtypes): partials of
+// max |a_i - b_i|, Re/Im of sum conj(a_i) b_i and sum | |a_i|^2 - |b_i|^2 |.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kThreads) void k_compare(const cplx<TA> *__restrict__ a, const cplx<TB> *__restrict__ b,
+                                                      uint64_t count, double *__restrict__ partials) {
+    __shared__ double scratch[kThreads / 64];
+    __shared__ double mx[kThreads];
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    double m = 0.0, sr = 0.0, si = 0.0, tv = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const double ar = double(a[i].re), ai = double(a[i].im);
+        const double br = double(b[i].re), bi = double(b[i].im);
+        const double dr = ar - br, di = ai - bi;
+        m = fmax(m, sqrt(dr * dr + di * di));
+        sr += ar * br + ai * bi;
+        si += ar * bi - ai * br;
+        tv += fabs((ar * ar + ai * ai) - (br * br + bi * bi));
+    }
+    mx[threadIdx.x] = m;
+    const double r0 = block_sum(sr, scratch);
+    const double r1 = block_sum(si, scratch);
+    const double r2 = block_sum(tv, scratch);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int t = 0; t < kThreads; ++t) mm = fmax(mm, mx[t]);
+        partials[4 * size_t(blockIdx.x)] = mm;
+        partials[4 * size_t(blockIdx.x) + 1] = r0;
+        partials[4 * size_t(blockIdx.x) + 2] = r1;
+        partials[4 * size_t(blockIdx.x) + 3] = r2;
+    }
+}
+
 // Full distribution over all n qubits in a given order: out[j(i)] = |psi_i|^2.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_probs_perm(const cplx<T> *__restrict__ amps, int n,
@@ -452,6 +484,7 @@ struct dq_state {
     double *d_result = nullptr;
     // stats
     uint64_t sweeps = 0, launches = 0, gates = 0;
+    uint64_t fused_blocks = 0, fused_gates = 0;  // fusion level 3 statistics
     // optional per-launch timing of the sweep kernels (HIP events on `stream`)
     int timing = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
@@ -682,7 +715,7 @@ int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *ma
     // fusion level 1 keeps the reference's rounding: the exact kernel;
     // level 2 already reorders commuting ops and uses the contracted kernel.
     auto *fn = std::is_same<T, double>::value
-                   ? (s->fuse == 2 ? dq_launch_tile_reg_f64_fma : dq_launch_tile_reg_f64)
+                   ? (s->fuse >= 2 ? dq_launch_tile_reg_f64_fma : dq_launch_tile_reg_f64)
                    : dq_launch_tile_reg_f32;
     hipError_t e = fn(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles, s->d_partials);
     s->last_grid = int(grid);
@@ -703,6 +736,8 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     size_t total_ops = 0;
     for (size_t i = 0; i < passes.size(); ++i) {
         plan_pass(&cfg, s->queue, passes[i], mats, plans[i]);
+        s->fused_blocks += uint64_t(plans[i].fused_blocks);
+        s->fused_gates += uint64_t(plans[i].fused_gates);
         plans[i].pd.op_begin = int(total_ops);
         plans[i].pd.op_count = int(plans[i].ops.size());
         plans[i].pd.red_bit = -1;
@@ -1318,13 +1353,48 @@ int dq_pauli_inner(dq_state *bra, dq_state *ket, int n_terms, const uint64_t *xm
     return DQ_OK;
 }
 
+int dq_compare_states(dq_state *a, dq_state *b, double *max_abs, double *inner_re, double *inner_im,
+                      double *tvd) {
+    if (!a || !b || !max_abs || !inner_re || !inner_im || !tvd) return fail(DQ_ERR_ARG, "null argument");
+    if (a->n != b->n) return fail(DQ_ERR_ARG, "states differ in size");
+    if (int rc = join(a, b)) return rc;
+    const uint64_t count = amp_count(a);
+    const int blocks = std::min<int>(kReduceBlocks, grid_for(count));
+#define DQ_CMP(TA, TB)                                                                                   \
+    hipLaunchKernelGGL((k_compare<TA, TB>), dim3(blocks), dim3(kThreads), 0, a->stream,                  \
+                       static_cast<const cplx<TA> *>(a->amps), static_cast<const cplx<TB> *>(b->amps), count, \
+                       a->d_partials)
+    if (a->dtype == DQ_C128 && b->dtype == DQ_C128) DQ_CMP(double, double);
+    else if (a->dtype == DQ_C128) DQ_CMP(double, float);
+    else if (b->dtype == DQ_C128) DQ_CMP(float, double);
+    else DQ_CMP(float, float);
+#undef DQ_CMP
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(4 * size_t(blocks));
+    HIP_TRY(hipMemcpyAsync(part.data(), a->d_partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    double m = 0, r = 0, i = 0, t = 0;
+    for (int k = 0; k < blocks; ++k) {
+        m = std::max(m, part[4 * size_t(k)]);
+        r += part[4 * size_t(k) + 1];
+        i += part[4 * size_t(k) + 2];
+        t += part[4 * size_t(k) + 3];
+    }
+    *max_abs = m;
+    *inner_re = r;
+    *inner_im = i;
+    *tvd = 0.5 * t;
+    return DQ_OK;
+}
+
 int dq_set_options(dq_state *s, int fuse, int tile_bits, int low_bits) {
     if (!s) return fail(DQ_ERR_ARG, "null state");
     if (int rc = flush(s)) return rc;
     const int max_bits = 13;
     if (tile_bits != 0 && (tile_bits < 6 || tile_bits > max_bits))
         return fail(DQ_ERR_ARG, "tile_bits out of range");
-    if (fuse < 0 || fuse > 2) return fail(DQ_ERR_ARG, "fusion level must be 0, 1 or 2");
+    if (fuse < 0 || fuse > 3) return fail(DQ_ERR_ARG, "fusion level must be 0, 1, 2 or 3");
+    if (fuse == 3 && s->dtype != DQ_C64) return fail(DQ_ERR_ARG, "fusion level 3 (MFMA blocks) needs a complex64 state");
     if (low_bits != 0 && (low_bits < 1 || low_bits > 8)) return fail(DQ_ERR_ARG, "low_bits out of range");
     s->fuse = fuse;
     if (tile_bits) s->tile_bits = tile_bits;
@@ -1356,6 +1426,13 @@ int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches, uint64
     if (sweeps) *sweeps = s->sweeps;
     if (launches) *launches = s->launches;
     if (gates) *gates = s->gates;
+    return DQ_OK;
+}
+
+int dq_get_fused_stats(const dq_state *s, uint64_t *blocks, uint64_t *gates) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (blocks) *blocks = s->fused_blocks;
+    if (gates) *gates = s->fused_gates;
     return DQ_OK;
 }
 

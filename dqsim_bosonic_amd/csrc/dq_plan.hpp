@@ -4,8 +4,10 @@
 // into LDS-tiled sweeps, and the CPU test-suite exercises it directly.
 #pragma once
 #include <algorithm>
+#include <complex>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <vector>
 
 #include "dq_types.hpp"
@@ -166,7 +168,7 @@ inline std::vector<Pass> schedule(const PlanConfig *s, const std::vector<HostOp>
             const HostOp &h = queue[idx];
             const uint64_t qm = h.qmask();
             const bool diag = anywhere && is_diagonal(h);
-            const uint64_t block = (diag && s->fuse == 2) ? nd_block : (nd_block | d_block);
+            const uint64_t block = (diag && s->fuse >= 2) ? nd_block : (nd_block | d_block);
             if (qm & block) {
                 (diag ? d_block : nd_block) |= qm;
                 rest.push_back(idx);
@@ -189,6 +191,131 @@ inline std::vector<Pass> schedule(const PlanConfig *s, const std::vector<HostOp>
         remaining.swap(rest);
     }
     return passes;
+}
+
+// ---- gate fusion into 5-qubit blocks (fusion level 3, complex64 tiles) ----
+
+using cd = std::complex<double>;
+
+// Full 2^k x 2^k matrix of a queued op (qubits h.q, q[0] = MSB).
+inline std::vector<cd> op_full_matrix(const HostOp &h) {
+    const int dim = 1 << h.k;
+    std::vector<cd> m(size_t(dim) * dim, cd(0, 0));
+    if (h.kind == K_CTRL1 && h.dense.size() == 8 && h.k > 1) {  // controls as given to dq_apply_1q
+        for (int r = 0; r < dim; ++r) m[size_t(r) * dim + r] = cd(1, 0);
+        for (int base = 0; base < dim; base += 2) {
+            bool on = true;
+            for (int j = 0; j < h.k - 1; ++j) {
+                const int bitv = (base >> (h.k - 1 - j)) & 1;
+                const int want = (h.cval >> h.q[j]) & 1;
+                if (bitv != want) on = false;
+            }
+            if (!on) continue;
+            for (int r = 0; r < 2; ++r)
+                for (int c = 0; c < 2; ++c)
+                    m[size_t(base + r) * dim + base + c] = cd(h.dense[2 * (2 * r + c)], h.dense[2 * (2 * r + c) + 1]);
+        }
+        return m;
+    }
+    for (size_t e = 0; e < m.size(); ++e) m[e] = cd(h.dense[2 * e], h.dense[2 * e + 1]);
+    return m;
+}
+
+// M <- E(G) M, with E(G) the op's matrix embedded on the ordered qubit list qs
+// (qs[0] = MSB of the 2^m index).
+inline void embed_apply(std::vector<cd> &M, const std::vector<int> &qs, const HostOp &h) {
+    const int m = int(qs.size()), dim = 1 << m, gk = h.k, gdim = 1 << gk;
+    const std::vector<cd> G = op_full_matrix(h);
+    int gpos[5];
+    for (int j = 0; j < gk; ++j)
+        for (int t = 0; t < m; ++t)
+            if (qs[t] == h.q[j]) gpos[j] = m - 1 - t;  // bit position in the 2^m index
+    std::vector<cd> out(M.size(), cd(0, 0));
+    for (int r = 0; r < dim; ++r) {
+        int gr = 0, rest = r;
+        for (int j = 0; j < gk; ++j) {
+            gr = (gr << 1) | ((r >> gpos[j]) & 1);
+            rest &= ~(1 << gpos[j]);
+        }
+        for (int gc = 0; gc < gdim; ++gc) {
+            const cd g = G[size_t(gr) * gdim + gc];
+            if (g == cd(0, 0)) continue;
+            int k = rest;
+            for (int j = 0; j < gk; ++j)
+                if ((gc >> (gk - 1 - j)) & 1) k |= 1 << gpos[j];
+            for (int c = 0; c < dim; ++c) out[size_t(r) * dim + c] += g * M[size_t(k) * dim + c];
+        }
+    }
+    M.swap(out);
+}
+
+constexpr int kFuseMinQubits = 3;  // fused blocks narrower than this stay per-gate
+constexpr int kFuseMinOps = 4;     // ... and so do blocks with fewer gates
+
+// Rewrites a pass's op list into 5-qubit blocks (fusion level 3).  Greedy, in
+// rounds: a round walks the remaining ops in order and takes every tile op
+// that fits the open block (union of qubits <= 5) and touches no "frozen"
+// qubit; an op that is not taken freezes its qubits, so any later op that
+// would have to follow it stays behind too.  Every op taken therefore
+// commutes with every op it overtakes (disjoint qubits).  Blocks of at least
+// kFuseMinQubits qubits and kFuseMinOps gates become one K_FUSED op (their
+// FP64 product, stored in `store`); smaller blocks keep their gates.
+inline std::vector<const HostOp *> fuse_blocks(const std::vector<HostOp> &queue, const Pass &p, const int *pos,
+                                               const int *tq, int b, std::deque<HostOp> &store) {
+    std::vector<const HostOp *> out, rest, next;
+    for (int idx : p.ops) rest.push_back(&queue[idx]);
+    auto tile_op = [&](const HostOp &h) {
+        if (h.kind == K_COLLAPSE || h.k > 5) return false;
+        for (int j = 0; j < h.k; ++j)
+            if (pos[h.q[j]] >= kOuterBit) return false;
+        return true;
+    };
+    while (!rest.empty()) {
+        size_t lead = 0;  // ops outside the tile at the front go out as they are
+        while (lead < rest.size() && !tile_op(*rest[lead])) out.push_back(rest[lead++]);
+        if (lead == rest.size()) break;
+        std::vector<const HostOp *> blk;
+        uint64_t bq = 0, frozen = 0;
+        next.clear();
+        for (size_t i = lead; i < rest.size(); ++i) {
+            const HostOp &h = *rest[i];
+            const uint64_t qm = h.qmask();
+            if (tile_op(h) && !(qm & frozen) && __builtin_popcountll(bq | qm) <= 5) {
+                blk.push_back(&h);
+                bq |= qm;
+            } else {
+                next.push_back(&h);
+                frozen |= qm;
+            }
+        }
+        if (__builtin_popcountll(bq) >= kFuseMinQubits && int(blk.size()) >= kFuseMinOps) {
+            std::vector<int> qs;
+            for (const HostOp *h : blk)
+                for (int j = 0; j < h->k; ++j)
+                    if (std::find(qs.begin(), qs.end(), h->q[j]) == qs.end()) qs.push_back(h->q[j]);
+            for (int t = b - 1; t >= 0 && qs.size() < 5; --t)  // pad with other tile qubits
+                if (std::find(qs.begin(), qs.end(), tq[t]) == qs.end()) qs.push_back(tq[t]);
+            std::vector<cd> M(32 * 32, cd(0, 0));
+            for (int r = 0; r < 32; ++r) M[size_t(r) * 32 + r] = cd(1, 0);
+            for (const HostOp *h : blk) embed_apply(M, qs, *h);
+            HostOp f;
+            f.kind = K_FUSED;
+            f.k = 5;
+            f.outcome = int(blk.size());  // gates in the block (statistics only)
+            for (int j = 0; j < 5; ++j) f.q[j] = qs[j];
+            f.dense.resize(2 * 32 * 32);
+            for (size_t e = 0; e < M.size(); ++e) {
+                f.dense[2 * e] = M[e].real();
+                f.dense[2 * e + 1] = M[e].imag();
+            }
+            store.push_back(std::move(f));
+            out.push_back(&store.back());
+        } else {
+            out.insert(out.end(), blk.begin(), blk.end());
+        }
+        rest.swap(next);
+    }
+    return out;
 }
 
 // ---- register-resident tile passes (k_tile_reg) ----
@@ -246,6 +373,8 @@ struct RegPlan {
     TilePass pd;
     int b;
     std::vector<TileOp> ops;
+    int fused_blocks = 0;  // K_FUSED ops formed in this pass
+    int fused_gates = 0;   // gates they replace
 };
 
 // Plans the layout of every op of one pass and encodes the ops.
@@ -272,10 +401,21 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     // the low qubits stay thread bits at load/store (coalesced runs)
     const unsigned lowmask = (1u << std::max(0, std::min(s->low_bits, b - 4))) - 1;
     const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - 4)) - 1);
-    const size_t nops = p.ops.size();
+    std::deque<HostOp> fused_store;
+    std::vector<const HostOp *> eff;
+    if (s->fuse == 3 && b >= 12) {
+        eff = fuse_blocks(queue, p, pos, pd.tq, b, fused_store);
+        for (const HostOp &f : fused_store) {
+            ++out.fused_blocks;
+            out.fused_gates += f.outcome;
+        }
+    } else {
+        for (int idx : p.ops) eff.push_back(&queue[idx]);
+    }
+    const size_t nops = eff.size();
     std::vector<unsigned> hard(nops, 0), soft(nops, 0);
     for (size_t i = 0; i < nops; ++i) {
-        const HostOp &h = queue[p.ops[i]];
+        const HostOp &h = *eff[i];
         unsigned qm = 0;
         bool all_inner = true;
         for (int j = 0; j < h.k; ++j) {
@@ -338,7 +478,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     pd.store_rb = (unpack_rb(last) & lowmask) ? pack_rb(dflt) : last;
 
     for (size_t i = 0; i < nops; ++i) {
-        const HostOp &h = queue[p.ops[i]];
+        const HostOp &h = *eff[i];
         TileOp t;
         std::memset(&t, 0, sizeof t);
         t.k = h.k;
@@ -413,6 +553,10 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
             t.kind = T_COLLAPSE;
             t.outcome = h.outcome;
             t.norm = h.norm;
+            break;
+        case K_FUSED:
+            t.kind = T_FUSED5;
+            m = &h.dense;
             break;
         default:
             break;

@@ -267,6 +267,66 @@ __device__ __forceinline__ void reg_diag_split(cplx<T> (&a)[16], cplx<T> d0, cpl
     for (int j = 0; j < 16; ++j) a[j] = cmul(((j >> P) & 1) ? d1 : d0, a[j]);
 }
 
+// Fused 5-qubit unitary on FP32 MFMA (complex64 tiles).  The tile, in LDS,
+// is a 32 x 2^(B-5) complex matrix V (rows: the 5 gate bits, bit[0] = MSB;
+// columns: the other tile bits).  W = U V is computed as the real product
+//   [Wr; Wi] = [[Ur, -Ui], [Ui, Ur]] [Vr; Vi]
+// with v_mfma_f32_32x32x2f32, the K index ordered (gate index s, re/im h):
+// lane l supplies B = (h ? Im : Re) V[s][c] with h = l / 32, c = its column,
+// and A = U[i][s] recombined per row block, i = l % 32.  Wave w owns columns
+// [32 w, 32 w + 32): it reads and writes only those, so no barrier is needed
+// between its reads and writes.  Output layout of the 32x32 MFMA (checked on
+// the device): D[row][col] with col = lane % 32 and, for accumulator v,
+// row = 8 (v / 4) + 4 (lane / 32) + v % 4.
+typedef float dq_floatx16 __attribute__((ext_vector_type(16)));
+
+template <int B>
+__device__ __forceinline__ void fused5_mfma(cplx<float> *lds, const cplx<float> *__restrict__ u, const TileOp &op,
+                                            unsigned t) {
+    const unsigned lane = t & 63, wave = t >> 6;
+    const unsigned i = lane & 31, h = lane >> 5;
+    unsigned fmask = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) fmask |= 1u << op.bit[b];
+    // column c = 32 * wave + i, scattered onto the non-gate tile bits
+    const unsigned c = 32 * wave + i;
+    unsigned cpart = 0;
+    {
+        int cb = 0;
+#pragma unroll
+        for (int bpos = 0; bpos < B; ++bpos) {
+            if ((fmask >> bpos) & 1) continue;
+            if ((c >> cb) & 1) cpart |= 1u << bpos;
+            ++cb;
+        }
+    }
+    const unsigned sc = swz<B>(cpart);
+    dq_floatx16 d0 = {0}, d1 = {0};
+#pragma unroll 4
+    for (int s = 0; s < 32; ++s) {
+        unsigned gpart = 0;
+#pragma unroll
+        for (int b = 0; b < 5; ++b)
+            if ((s >> (4 - b)) & 1) gpart |= 1u << op.bit[b];
+        const cplx<float> v = lds[sc ^ swz<B>(gpart)];
+        const cplx<float> us = u[i * 32 + s];
+        const float bv = h ? v.im : v.re;
+        const float a0 = h ? -us.im : us.re;  // rows of Wr
+        const float a1 = h ? us.re : us.im;   // rows of Wi
+        d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bv, d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bv, d1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int row = 8 * (v / 4) + 4 * int(h) + (v % 4);
+        unsigned gpart = 0;
+#pragma unroll
+        for (int b = 0; b < 5; ++b)
+            if ((row >> (4 - b)) & 1) gpart |= 1u << op.bit[b];
+        lds[sc ^ swz<B>(gpart)] = cplx<float>{d0[v], d1[v]};
+    }
+}
+
 // One HBM sweep with register-resident amplitudes (see the file header).
 // The next tile's amplitudes are loaded into registers before the current
 // tile's ops run, so each tile's HBM latency overlaps the previous tile's work.
@@ -403,7 +463,7 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 }
                 break;
             case T_COLLAPSE: {
-                const T nrm = T(op.norm);
+                const double nrm = op.norm;  // renormalise in FP64 (exact division for T = double)
                 const int bp = op.bit[0];
                 const bool outer = bp >= kOuterBit;
                 const int ov = outer ? int((tile >> (bp - kOuterBit)) & 1) : 0;
@@ -412,14 +472,25 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                     const unsigned i = L.tpart | L.jp[j];
                     const int v = outer ? ov : int((i >> (bp & 31)) & 1);
                     if (v == op.outcome) {
-                        a[j].re = a[j].re / nrm;
-                        a[j].im = a[j].im / nrm;
+                        a[j].re = T(double(a[j].re) / nrm);
+                        a[j].im = T(double(a[j].im) / nrm);
                     } else {
                         a[j] = cplx<T>{T(0), T(0)};
                     }
                 }
                 break;
             }
+            case T_FUSED5:
+                if constexpr (sizeof(T) == 4 && B >= 12) {
+                    __syncthreads();
+                    to_lds<T, B>(lds, a, L);
+                    __syncthreads();
+                    fused5_mfma<B>(reinterpret_cast<cplx<float> *>(lds), reinterpret_cast<const cplx<float> *>(m),
+                                   op, t);
+                    __syncthreads();
+                    from_lds<T, B>(a, lds, L);
+                }
+                break;
             default: {  // T_LDSOP
                 __syncthreads();
                 to_lds<T, B>(lds, a, L);

@@ -10,7 +10,8 @@ enum OpKind : int {
     K_MONO = 2,     // monomial: out[r] = ph[r] * in[perm[r]], ph at mat, perm at aux
     K_CTRL1 = 3,    // 2x2 on bit[k-1] where (i & cmask) == cval
     K_DENSEK = 4,   // dense 2^k x 2^k, acc = 0; acc += U[r][c] * v[c]
-    K_COLLAPSE = 5  // keep bit[0] == outcome (divide by norm), zero the rest
+    K_COLLAPSE = 5,  // keep bit[0] == outcome (divide by norm), zero the rest
+    K_FUSED = 6      // host only: a fused block of gates on 5 qubits (dense 32 x 32)
 };
 
 // One queued operation, positions relative to the index space it runs on
@@ -36,8 +37,10 @@ enum TileKind : int {
     T_DIAG = 3,      // diagonal 2^k on any tile bits, elementwise
     T_COLLAPSE = 4,  // measurement collapse on tile bit bit[0], elementwise
     T_LDSOP = 5,     // any 2^k x 2^k matrix through LDS, one output per amplitude
-    T_DIAGT = 6      // diagonal, k <= 2: outer bits, thread bits and at most one register bit
+    T_DIAGT = 6,     // diagonal, k <= 2: outer bits, thread bits and at most one register bit
                      // (slot p, gate-index bit q)
+    T_FUSED5 = 7     // complex64 only: a fused 5-qubit unitary (32x32) on tile bits, as a
+                     // real 64x64 GEMM on FP32 MFMA over the tile's 32-amplitude columns
 };
 
 // Matrix shape of a register op, chosen on the host.  Every variant computes

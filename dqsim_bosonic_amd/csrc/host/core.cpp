@@ -38,6 +38,7 @@
 #include "circuit.hpp"
 #include "gates.hpp"
 #include "rng.hpp"
+#include "../dq_plan.hpp"
 
 namespace py = pybind11;
 using namespace dqh;
@@ -396,7 +397,8 @@ class StatevectorSimulator {
 // Used by the benchmark (one step = reset to |0...0> + the whole circuit).
 class Engine {
    public:
-    Engine(int n, int fusion, int tile_bits, int low_bits) : n_(n), st_(std::make_shared<DeviceState>(n)) {
+    Engine(int n, int fusion, int tile_bits, int low_bits, int dtype)
+        : n_(n), st_(std::make_shared<DeviceState>(n, dtype)) {
         check(dq_set_options(st_->s, fusion, tile_bits, low_bits), "options");
     }
     void load(const std::string &text) {
@@ -427,6 +429,23 @@ class Engine {
         uint64_t a = 0, b = 0, c = 0;
         check(dq_get_stats(st_->s, &a, &b, &c), "stats");
         return py::make_tuple(a, b, c);
+    }
+    py::tuple fused_stats() {
+        uint64_t a = 0, b = 0;
+        check(dq_get_fused_stats(st_->s, &a, &b), "fused stats");
+        return py::make_tuple(a, b);
+    }
+    // Distance between this engine's state and another's (any precisions).
+    py::dict compare(Engine &other) {
+        if (other.n_ != n_) throw py::value_error("engines differ in width");
+        double m = 0, r = 0, i = 0, t = 0;
+        check(dq_compare_states(st_->s, other.st_->s, &m, &r, &i, &t), "compare");
+        py::dict d;
+        d["max_abs"] = m;
+        d["inner"] = std::complex<double>(r, i);
+        d["fidelity"] = r * r + i * i;
+        d["tvd"] = t;
+        return d;
     }
     SimulationResult result() const { return SimulationResult(st_, n_, {}, py::none()); }
 
@@ -716,13 +735,15 @@ PYBIND11_MODULE(_core, m) {
         py::arg("profile") = false);
 
     py::class_<Engine>(m, "Engine")
-        .def(py::init<int, int, int, int>(), py::arg("n"), py::arg("fusion") = 2, py::arg("tile_bits") = 0,
-             py::arg("low_bits") = 0)
+        .def(py::init<int, int, int, int, int>(), py::arg("n"), py::arg("fusion") = 2, py::arg("tile_bits") = 0,
+             py::arg("low_bits") = 0, py::arg("dtype") = 0)
         .def("load", &Engine::load)
         .def("step", &Engine::step, py::arg("seed"))
         .def("set_timing", &Engine::set_timing)
         .def("timing", &Engine::timing)
         .def("stats", &Engine::stats)
+        .def("fused_stats", &Engine::fused_stats)
+        .def("compare", &Engine::compare)
         .def("result", &Engine::result);
 
     py::class_<Adjoint>(m, "Adjoint")
@@ -770,6 +791,62 @@ PYBIND11_MODULE(_core, m) {
         };
         for (const Instr &in : c.instructions) out.append(conv(in));
         return py::make_tuple(c.num_qubits(), c.num_cbits(), out);
+    });
+    // Fusion level 3's block former on a plain op list (qubits not in
+    // tile_qubits are outer).  ops: (qubits, flat complex matrix, control
+    // values or empty); with control values the matrix is the 2x2 applied to
+    // qubits[-1] under controls qubits[:-1] (as dq_apply_1q).  Returns, in
+    // order, (qubits, 32x32 matrix, gate count) for a fused block or
+    // (original index,) for an op kept as it is.
+    m.def("_fuse_blocks", [](std::vector<int> tile_qubits, py::list ops) {
+        std::vector<dq::HostOp> queue;
+        for (py::handle item : ops) {
+            py::tuple t = py::reinterpret_borrow<py::tuple>(item);
+            auto qs = t[0].cast<std::vector<int>>();
+            auto u = t[1].cast<std::vector<std::complex<double>>>();
+            auto cv = t[2].cast<std::vector<int>>();
+            dq::HostOp op;
+            op.k = int(qs.size());
+            for (int j = 0; j < op.k; ++j) op.q[j] = qs[size_t(j)];
+            std::vector<double> flat(2 * u.size());
+            for (size_t e = 0; e < u.size(); ++e) {
+                flat[2 * e] = u[e].real();
+                flat[2 * e + 1] = u[e].imag();
+            }
+            if (!cv.empty()) {
+                op.kind = dq::K_CTRL1;
+                for (int j = 0; j + 1 < op.k; ++j) {
+                    op.cmask |= uint64_t(1) << qs[size_t(j)];
+                    if (cv[size_t(j)]) op.cval |= uint64_t(1) << qs[size_t(j)];
+                }
+                op.mat = flat;
+                op.dense = flat;
+            } else {
+                dq::classify(op, flat.data());
+            }
+            queue.push_back(std::move(op));
+        }
+        int pos[64], tq[16] = {0};
+        for (int q = 0; q < 64; ++q) pos[q] = dq::kOuterBit + q;
+        for (size_t t = 0; t < tile_qubits.size() && t < 16; ++t) {
+            pos[tile_qubits[t]] = int(t);
+            tq[t] = tile_qubits[t];
+        }
+        dq::Pass p;
+        for (size_t i = 0; i < queue.size(); ++i) p.ops.push_back(int(i));
+        std::deque<dq::HostOp> store;
+        auto eff = dq::fuse_blocks(queue, p, pos, tq, int(tile_qubits.size()), store);
+        py::list out;
+        for (const dq::HostOp *h : eff) {
+            if (h->kind == dq::K_FUSED) {
+                py::array_t<std::complex<double>> m({32, 32});
+                std::memcpy(m.mutable_data(), h->dense.data(), h->dense.size() * sizeof(double));
+                out.append(py::make_tuple(std::vector<int>(h->q, h->q + 5), m, h->outcome));
+            } else {
+                out.append(py::make_tuple(int(h - queue.data())));
+            }
+        }
+        return out;
     });
     m.def("_parse_mode", [](const std::string &mode) { return int(parse_mode(mode)); });
 }

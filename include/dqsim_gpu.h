@@ -107,6 +107,12 @@ int dq_apply_pauli_sum(dq_state *dst, dq_state *src, int n_terms, const uint64_t
 int dq_pauli_inner(dq_state *bra, dq_state *ket, int n_terms, const uint64_t *xmask,
                    const uint64_t *zmask, const double *coeff, double *re, double *im);
 
+/* Compares two states of the same size and any dtypes (the complex64 error
+ * report): max_i |a_i - b_i|, <a|b>, and the total-variation distance
+ * 0.5 * sum_i | |a_i|^2 - |b_i|^2 |, all accumulated in FP64. */
+int dq_compare_states(dq_state *a, dq_state *b, double *max_abs, double *inner_re,
+                      double *inner_im, double *tvd);
+
 /* Flushes queued gates and waits for the device. */
 int dq_synchronize(dq_state *s);
 
@@ -116,7 +122,10 @@ int dq_synchronize(dq_state *s);
  *             order (bit-identical to the reference's arithmetic);
  *   fuse = 2: as 1, but a sweep may also take later gates on qubits disjoint
  *             from every gate it skipped (default; identical in exact
- *             arithmetic, last-bit rounding may differ).
+ *             arithmetic, last-bit rounding may differ);
+ *   fuse = 3: complex64 states only: as 2, and runs of gates on at most 5
+ *             qubits are fused (FP64 product on the host) into 32x32 blocks
+ *             applied as GEMMs on FP32 MFMA.
  *   tile_bits: qubits per tile (6..13);
  *   low_bits: qubits 0..low_bits-1 are in every tile, so each tile is made of
  *             contiguous runs of 2^low_bits amplitudes (1..8). */
@@ -132,6 +141,10 @@ int dq_get_timing(dq_state *s, double *sweep_ms, uint64_t *sweep_launches);
 /* Number of HBM sweeps and kernel launches issued so far on this handle. */
 int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches,
                  uint64_t *gates);
+
+/* Fusion level 3: 5-qubit blocks applied on MFMA so far, and the gates they
+ * replaced (no reference counterpart). */
+int dq_get_fused_stats(const dq_state *s, uint64_t *blocks, uint64_t *gates);
 
 const char *dq_last_error(void);
 

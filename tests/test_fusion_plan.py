@@ -1,0 +1,104 @@
+"""CPU tests of fusion level 3's block former (no device needed).
+
+The planner groups gates into blocks on at most 5 qubits and multiplies each
+block into one 32x32 matrix in FP64; gates it overtakes must commute with the
+block.  The check is the mathematics: applying the rewritten list (fused
+blocks plus the gates left as they are) to a random state must give the same
+state as the original gate sequence, to FP64 rounding, computed with the
+oracle's gate application.  A wrong embedding order, a reordering across a
+shared qubit or a dropped gate fails by O(1).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+core = pytest.importorskip("dqsim_bosonic_amd._core")
+
+GATES_1Q = ["h", "s", "t", "sx", "u3", "rx", "ry", "rz", "x"]
+GATES_2Q = ["cx", "cz", "cp", "rzz", "rxx", "swap", "cry"]
+NPARAM = {"u3": 3, "rx": 1, "ry": 1, "rz": 1, "cp": 1, "rzz": 1, "rxx": 1, "cry": 1}
+
+
+def random_ops(n, count, rng):
+    ops = []
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.45:
+            g = GATES_1Q[int(rng.integers(len(GATES_1Q)))]
+            ops.append(([int(rng.integers(n))], orc.gate_matrix(g, rng.uniform(-4, 4, NPARAM.get(g, 0))), []))
+        elif r < 0.8:
+            g = GATES_2Q[int(rng.integers(len(GATES_2Q)))]
+            qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+            ops.append((qs, orc.gate_matrix(g, rng.uniform(-4, 4, NPARAM.get(g, 0))), []))
+        elif r < 0.9:
+            qs = [int(x) for x in rng.choice(n, size=3, replace=False)]
+            ops.append((qs, orc.gate_matrix("ccx", []), []))
+        else:
+            qs = [int(x) for x in rng.choice(n, size=3, replace=False)]
+            ops.append((qs, orc.gate_matrix("u3", rng.uniform(-4, 4, 3)), [int(rng.integers(2)), int(rng.integers(2))]))
+    return ops
+
+
+def apply_op(psi, n, qs, u, cv):
+    if not cv:
+        if len(qs) == 1:
+            orc.apply_one_qubit(psi, n, u, qs[0])
+        else:
+            orc.apply_n_qubit(psi, n, u, qs)
+        return psi
+    idx = np.arange(1 << n)
+    mask = np.ones(1 << n, dtype=bool)
+    for q, v in zip(qs[:-1], cv):
+        mask &= ((idx >> q) & 1) == v
+    sub = psi.copy()
+    orc.apply_one_qubit(sub, n, u, qs[-1])
+    return np.where(mask, sub, psi)
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("n,tile", [(8, list(range(8))), (9, [0, 1, 2, 3, 5, 7, 8]), (10, [9, 8, 7, 6, 5, 4])])
+def test_fused_list_equals_program(seed, n, tile):
+    rng = np.random.default_rng(seed)
+    ops = random_ops(n, 60, rng)
+    plan = core._fuse_blocks(tile, [(qs, u.ravel(), cv) for qs, u, cv in ops])
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    want = psi.copy()
+    for qs, u, cv in ops:
+        want = apply_op(want, n, qs, u, cv)
+    got = psi.copy()
+    seen, fused = [], 0
+    for item in plan:
+        if len(item) == 1:
+            qs, u, cv = ops[item[0]]
+            got = apply_op(got, n, qs, u, cv)
+            seen.append(item[0])
+        else:
+            qs, m, cnt = item
+            assert len(set(qs)) == 5 and all(q in tile for q in qs)
+            assert np.allclose(m.conj().T @ m, np.eye(32), atol=1e-12)
+            orc.apply_n_qubit(got, n, m, qs)
+            fused += cnt
+    assert len(seen) + fused == len(ops)
+    assert len(set(seen)) == len(seen)
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def test_blocks_form_on_layered_circuit():
+    """A brickwork of 1-qubit layers and neighbour cz on 12 tile qubits
+    fuses most gates into blocks of at least 4 gates."""
+    n = 12
+    rng = np.random.default_rng(1)
+    ops = []
+    for layer in range(6):
+        for q in range(n):
+            ops.append(([q], orc.gate_matrix("u3", rng.uniform(-4, 4, 3)).ravel(), []))
+        for q in range(layer % 2, n - 1, 2):
+            ops.append(([q, q + 1], orc.gate_matrix("cz", []).ravel(), []))
+    plan = core._fuse_blocks(list(range(n)), ops)
+    fused = sum(item[2] for item in plan if len(item) == 3)
+    blocks = sum(1 for item in plan if len(item) == 3)
+    assert blocks > 0 and fused >= 0.8 * len(ops), (blocks, fused, len(ops))
