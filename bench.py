@@ -35,6 +35,11 @@ on <= 5 qubits multiplied into 32x32 blocks in FP64 and applied on FP32 MFMA
 inside the LDS-tiled sweep) and as complex128 (fusion level 2) on the same GPU
 (32 GiB + 64 GiB).  Reported: gates/s of each, and on device max |d amp|,
 1 - |<psi64|psi32>|^2 and the total variation distance of the probabilities.
+
+batch (batching form): 4096 hardware-efficient-ansatz circuits, n ~ U{10..24},
+depth ~ U{8..64} (ry, rz on every qubit then a cx ring per layer), <sum_i
+Z_i Z_{i+1}> of each, complex128 and complex64: circuits/s, and the unfused
+algorithmic bytes (every gate reading and writing the whole state) per second.
 """
 from __future__ import annotations
 
@@ -130,6 +135,39 @@ def adjoint_leg(n: int, layers: int, seed: int, ps_subset: int) -> dict:
             "c64_rel_err_vs_f64": rel32, "c64_energy_err": abs(e32 - e)}
 
 
+def batch_leg(count: int, seed: int, workers: int) -> dict:
+    """4096 HEA circuits (n ~ U{10..24}, d ~ U{8..64}) -> <sum Z_i Z_i+1> each."""
+    import numpy as np
+
+    from dqsim_bosonic_amd import _core
+    from dqsim_bosonic_amd import workloads as W
+
+    ns, ds, angles = W.hea_batch_arrays(count, seed)
+    flat = np.concatenate([a.ravel() for a in angles])
+    # unfused algorithmic bytes: every gate reads and writes every amplitude once
+    gates = ds * 3 * ns
+    alg = {"c128": float(np.sum(gates * 2.0 * 16.0 * np.exp2(ns))),
+           "c64": float(np.sum(gates * 2.0 * 8.0 * np.exp2(ns)))}
+    out = {"workload": f"{count} HEA circuits, n ~ U{{10..24}}, depth ~ U{{8..64}}, seed {seed}; "
+                       "<sum_i Z_i Z_i+1> each", "gates": int(np.sum(gates))}
+    _core.simulate_hea_batch(ns[:64], ds[:64], np.concatenate([a.ravel() for a in angles[:64]]),
+                             dtype="c128", workers=workers)  # warm-up (kernel loads, allocations)
+    res = {}
+    for dt in ("c128", "c64"):
+        t0 = time.perf_counter()
+        e, st = _core.simulate_hea_batch(ns, ds, flat, dtype=dt, workers=workers)
+        el = time.perf_counter() - t0
+        res[dt] = e
+        out[dt] = {"circuits_per_s": round(count / el, 2), "seconds": round(el, 3),
+                   "gates_per_s": round(float(np.sum(gates)) / el, 1),
+                   "unfused_equiv_GBps": round(alg[dt] / el / 1e9, 1),
+                   "lds_instances": int(st["lds_instances"]), "engine_instances": int(st["engine_instances"]),
+                   "lds_gpu_ms": round(float(st["lds_gpu_ms"]), 3)}
+        log(f"[bench] batch {dt}: {el:.3f} s")
+    out["c64_max_abs_diff_vs_c128"] = float(np.max(np.abs(res["c64"] - res["c128"])))
+    return out
+
+
 def precision_leg(n: int, depth: int, seed: int, steps: int) -> dict:
     from dqsim_bosonic_amd import _core
     from dqsim_bosonic_amd import workloads as W
@@ -197,6 +235,9 @@ def main() -> int:
     ap.add_argument("--adjoint-layers", type=int, default=40)
     ap.add_argument("--ps-subset", type=int, default=8)
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--precision-qubits", type=int, default=32)
     ap.add_argument("--precision-depth", type=int, default=60)
     args = ap.parse_args()
@@ -262,6 +303,12 @@ def main() -> int:
         prec = precision_leg(args.precision_qubits, args.precision_depth, args.seed, 1)
         log(f"[bench] precision {prec}")
 
+    batch = None
+    if not args.no_batch:
+        log("[bench] batch (HEA circuits, <ZZ>) ...")
+        batch = batch_leg(args.batch_count, args.seed, args.batch_workers)
+        log(f"[bench] batch {batch}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -280,6 +327,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "adjoint": adj,
         "precision": prec,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -29,12 +29,14 @@ from ._core import (  # noqa: E402
     SimulationProfile,
     SimulationResult,
     StatevectorSimulator,
+    simulate_batch,
     simulate_monolithic,
 )
 
 ENGINE_LIBRARY = _os.path.join(_HERE, "libdqsim_gpu.so")
 
 __all__ = [
+    "simulate_batch",
     "StatevectorSimulator",
     "SimulationResult",
     "SimulationProfile",

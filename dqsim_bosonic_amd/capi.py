@@ -24,7 +24,9 @@ EXPORTS = (
     "dq_copy_to_host", "dq_synchronize", "dq_state_copy", "dq_apply_pauli_sum", "dq_pauli_inner",
     "dq_compare_states",
     "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
-    "dq_get_fused_stats",
+    "dq_get_fused_stats", "dq_set_fuse_min_cost",
+    "dq_batch_create", "dq_batch_destroy", "dq_batch_add_instance", "dq_batch_apply",
+    "dq_batch_set_observable", "dq_batch_set_workers", "dq_batch_run", "dq_batch_stats",
     "dq_last_error",
 )
 
@@ -58,6 +60,16 @@ def lib() -> ctypes.CDLL:
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.dq_get_stats.argtypes = [vp, u64p, u64p, u64p]
         L.dq_get_fused_stats.argtypes = [vp, u64p, u64p]
+        L.dq_set_fuse_min_cost.argtypes = [vp, ctypes.c_double]
+        L.dq_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.dq_batch_destroy.argtypes = [vp]
+        L.dq_batch_destroy.restype = None
+        L.dq_batch_add_instance.argtypes = [vp, ctypes.c_int, _ip]
+        L.dq_batch_apply.argtypes = [vp, ctypes.c_int, _dp, _ip, ctypes.c_int]
+        L.dq_batch_set_observable.argtypes = [vp, ctypes.c_int, ctypes.c_int, u64p, _dp]
+        L.dq_batch_set_workers.argtypes = [vp, ctypes.c_int]
+        L.dq_batch_run.argtypes = [vp, _dp]
+        L.dq_batch_stats.argtypes = [vp, u64p, u64p, _dp]
         L.dq_state_copy.argtypes = [vp, vp]
         L.dq_compare_states.argtypes = [vp, vp, _dp, _dp, _dp, _dp]
         L.dq_apply_pauli_sum.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp]
@@ -187,8 +199,56 @@ class State:
         _check(lib().dq_get_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    def set_fuse_min_cost(self, min_cost: float) -> None:
+        _check(lib().dq_set_fuse_min_cost(self.h, float(min_cost)))
+
     def fused_stats(self) -> tuple[int, int]:
         """(5-qubit MFMA blocks applied, gates they replaced) under fusion level 3."""
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().dq_get_fused_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+class Batch:
+    """Owning wrapper of a dq_batch handle (many independent circuits)."""
+
+    def __init__(self, dtype: int = C128):
+        h = ctypes.c_void_p()
+        _check(lib().dq_batch_create(dtype, ctypes.byref(h)))
+        self.h = h
+        self.count = 0
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.dq_batch_destroy(self.h)
+            self.h = None
+
+    def add(self, n: int) -> int:
+        i = ctypes.c_int()
+        _check(lib().dq_batch_add_instance(self.h, n, ctypes.byref(i)))
+        self.count += 1
+        return i.value
+
+    def apply(self, index: int, u: np.ndarray, qubits) -> None:
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        q = (ctypes.c_int * len(qubits))(*qubits)
+        _check(lib().dq_batch_apply(self.h, index, _d(u.view(np.float64)), q, len(qubits)))
+
+    def set_observable(self, index: int, terms) -> None:
+        """terms: (coeff, zmask) pairs."""
+        zm = (ctypes.c_uint64 * max(1, len(terms)))(*[int(t[1]) for t in terms])
+        c = np.ascontiguousarray([float(t[0]) for t in terms] or [0.0], dtype=np.float64)
+        _check(lib().dq_batch_set_observable(self.h, index, len(terms), zm, _d(c)))
+
+    def set_workers(self, w: int) -> None:
+        _check(lib().dq_batch_set_workers(self.h, w))
+
+    def run(self) -> np.ndarray:
+        out = np.zeros(max(1, self.count), dtype=np.float64)
+        _check(lib().dq_batch_run(self.h, _d(out)))
+        return out[: self.count]
+
+    def stats(self) -> tuple[int, int, float]:
+        a, b, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        _check(lib().dq_batch_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
+        return a.value, b.value, t.value

@@ -152,7 +152,10 @@ __device__ __forceinline__ void dense_k(cplx<T> *__restrict__ a, int nbits, cons
 
 // Applies one op to the 2^nbits amplitudes at `a` (LDS tile or global
 // state), work items distributed over `nthr` threads starting at `tid`.
-template <typename T, typename I>
+// Monomial and dense ops on more than KMAX qubits are skipped: a caller
+// with KMAX < 5 applies those itself (so their register arrays are not part
+// of its kernel).
+template <typename T, typename I, int KMAX = 5>
 __device__ __forceinline__ void apply_op(cplx<T> *__restrict__ a, int nbits, const DevOp &op,
                                          const cplx<T> *__restrict__ mats,
                                          const int *__restrict__ perms, I tid, I nthr) {
@@ -219,8 +222,8 @@ __device__ __forceinline__ void apply_op(cplx<T> *__restrict__ a, int nbits, con
         case 1: mono_k<T, I, 1>(a, nbits, op, mats, perms, tid, nthr); break;
         case 2: mono_k<T, I, 2>(a, nbits, op, mats, perms, tid, nthr); break;
         case 3: mono_k<T, I, 3>(a, nbits, op, mats, perms, tid, nthr); break;
-        case 4: mono_k<T, I, 4>(a, nbits, op, mats, perms, tid, nthr); break;
-        default: mono_k<T, I, 5>(a, nbits, op, mats, perms, tid, nthr); break;
+        case 4: if constexpr (KMAX >= 4) mono_k<T, I, 4>(a, nbits, op, mats, perms, tid, nthr); break;
+        default: if constexpr (KMAX >= 5) mono_k<T, I, 5>(a, nbits, op, mats, perms, tid, nthr); break;
         }
         break;
     case K_DENSEK:
@@ -228,8 +231,8 @@ __device__ __forceinline__ void apply_op(cplx<T> *__restrict__ a, int nbits, con
         case 1: dense_k<T, I, 1>(a, nbits, op, mats, tid, nthr); break;
         case 2: dense_k<T, I, 2>(a, nbits, op, mats, tid, nthr); break;
         case 3: dense_k<T, I, 3>(a, nbits, op, mats, tid, nthr); break;
-        case 4: dense_k<T, I, 4>(a, nbits, op, mats, tid, nthr); break;
-        default: dense_k<T, I, 5>(a, nbits, op, mats, tid, nthr); break;
+        case 4: if constexpr (KMAX >= 4) dense_k<T, I, 4>(a, nbits, op, mats, tid, nthr); break;
+        default: if constexpr (KMAX >= 5) dense_k<T, I, 5>(a, nbits, op, mats, tid, nthr); break;
         }
         break;
     default:

@@ -51,6 +51,14 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+}  // namespace
+
+// The one error channel of the library (dq_last_error), also used by the
+// batch entry points in dq_batch.hip.
+int dq::set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
         hipError_t _e = (expr);                                                        \
@@ -485,6 +493,7 @@ struct dq_state {
     // stats
     uint64_t sweeps = 0, launches = 0, gates = 0;
     uint64_t fused_blocks = 0, fused_gates = 0;  // fusion level 3 statistics
+    double fuse_min_cost = 128.0;                // fusion level 3 cost threshold
     // optional per-launch timing of the sweep kernels (HIP events on `stream`)
     int timing = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
@@ -504,6 +513,7 @@ PlanConfig plan_config(const dq_state *s) {
     c.tile_bits = s->tile_bits;
     c.low_bits = s->low_bits;
     c.reg_tiles = s->reg_tiles;
+    c.fuse_min_cost = s->fuse_min_cost;
     return c;
 }
 
@@ -1426,6 +1436,14 @@ int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches, uint64
     if (sweeps) *sweeps = s->sweeps;
     if (launches) *launches = s->launches;
     if (gates) *gates = s->gates;
+    return DQ_OK;
+}
+
+int dq_set_fuse_min_cost(dq_state *s, double min_cost) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (!(min_cost >= 0.0)) return fail(DQ_ERR_ARG, "fusion cost threshold must be >= 0");
+    if (int rc = flush(s)) return rc;
+    s->fuse_min_cost = min_cost;
     return DQ_OK;
 }
 

@@ -8,11 +8,15 @@
 #include <cstdint>
 #include <cstring>
 #include <deque>
+#include <string>
 #include <vector>
 
 #include "dq_types.hpp"
 
 namespace dq {
+
+// Sets the library's error message (dq_last_error) and returns code.
+int set_error(int code, const std::string &msg);
 
 struct PlanConfig {
     int n = 0;
@@ -20,6 +24,7 @@ struct PlanConfig {
     int tile_bits = 12;  // qubits per tile
     int low_bits = 5;    // qubits 0..low_bits-1 are in every tile (coalesced runs)
     int reg_tiles = 1;   // register-resident tiles for tiles of >= 9 qubits
+    double fuse_min_cost = 128.0;  // fusion level 3: least cost of a fused block's gates
 };
 
 inline bool dq_tile_bits_supported(int b) { return b == 9 || b == 12 || b == 13; }
@@ -252,16 +257,37 @@ inline void embed_apply(std::vector<cd> &M, const std::vector<int> &qs, const Ho
 constexpr int kFuseMinQubits = 3;  // fused blocks narrower than this stay per-gate
 constexpr int kFuseMinOps = 4;     // ... and so do blocks with fewer gates
 
+// Cost model of fusion level 3, in FP32 flops per amplitude.  A fused block
+// is a 32x32 complex GEMM: 32 complex MACs = 256 flops per amplitude on MFMA,
+// which runs at twice the rate of scalar FP32 VALU code, so it costs about
+// 128 VALU flops per amplitude.  The gates it replaces cost (VALU, per
+// amplitude): a dense 2x2 = 2 complex mul + 1 add = 14; a diagonal or
+// monomial = 1 complex mul = 6; a dense 2^k gate = 2^k complex MACs = 8 * 2^k;
+// a controlled 2x2 with c controls = 14 / 2^c.  A block is fused only when
+// the gates it replaces cost at least PlanConfig::fuse_min_cost (default 128).
+
+inline double gate_cost(const HostOp &h) {
+    switch (h.kind) {
+    case K_DENSE1: return 14.0;
+    case K_DIAG:
+    case K_MONO: return 6.0;
+    case K_CTRL1: return 14.0 / double(1 << (h.k - 1));
+    case K_DENSEK: return 8.0 * double(1 << h.k);
+    default: return 0.0;
+    }
+}
+
 // Rewrites a pass's op list into 5-qubit blocks (fusion level 3).  Greedy, in
 // rounds: a round walks the remaining ops in order and takes every tile op
 // that fits the open block (union of qubits <= 5) and touches no "frozen"
 // qubit; an op that is not taken freezes its qubits, so any later op that
 // would have to follow it stays behind too.  Every op taken therefore
 // commutes with every op it overtakes (disjoint qubits).  Blocks of at least
-// kFuseMinQubits qubits and kFuseMinOps gates become one K_FUSED op (their
-// FP64 product, stored in `store`); smaller blocks keep their gates.
+// kFuseMinQubits qubits and kFuseMinOps gates, whose gates cost at least
+// min_cost, become one K_FUSED op (their FP64 product, stored in
+// `store`); other blocks keep their gates.
 inline std::vector<const HostOp *> fuse_blocks(const std::vector<HostOp> &queue, const Pass &p, const int *pos,
-                                               const int *tq, int b, std::deque<HostOp> &store) {
+                                               const int *tq, int b, double min_cost, std::deque<HostOp> &store) {
     std::vector<const HostOp *> out, rest, next;
     for (int idx : p.ops) rest.push_back(&queue[idx]);
     auto tile_op = [&](const HostOp &h) {
@@ -288,7 +314,9 @@ inline std::vector<const HostOp *> fuse_blocks(const std::vector<HostOp> &queue,
                 frozen |= qm;
             }
         }
-        if (__builtin_popcountll(bq) >= kFuseMinQubits && int(blk.size()) >= kFuseMinOps) {
+        double cost = 0.0;
+        for (const HostOp *h : blk) cost += gate_cost(*h);
+        if (__builtin_popcountll(bq) >= kFuseMinQubits && int(blk.size()) >= kFuseMinOps && cost >= min_cost) {
             std::vector<int> qs;
             for (const HostOp *h : blk)
                 for (int j = 0; j < h->k; ++j)
@@ -404,7 +432,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     std::deque<HostOp> fused_store;
     std::vector<const HostOp *> eff;
     if (s->fuse == 3 && b >= 12) {
-        eff = fuse_blocks(queue, p, pos, pd.tq, b, fused_store);
+        eff = fuse_blocks(queue, p, pos, pd.tq, b, s->fuse_min_cost, fused_store);
         for (const HostOp &f : fused_store) {
             ++out.fused_blocks;
             out.fused_gates += f.outcome;

@@ -12,6 +12,10 @@ Each generator returns a ``circuit.Circuit`` in the reference's wire schema
   ``gate`` kind) between the two halves of the register; after every 10th
   layer, measure two random qubits, reset them, and apply one x conditioned
   on the first measured bit.
+* ``hea`` / ``hea_batch``: hardware-efficient ansatz circuits for batching:
+  per layer ry then rz on every qubit, then a cx ring cx(i, i+1), i = 0..n-2,
+  closed by cx(n-1, 0); angles U[0, 2pi).  ``hea_batch`` draws 4096 widths
+  n ~ U{10..24} and depths d ~ U{8..64} by default.
 """
 from __future__ import annotations
 
@@ -127,3 +131,35 @@ def random_mixed(n: int, count: int, seed: int, kinds=None) -> C.Circuit:
                 kw[f] = float(rng.uniform(-TWO_PI, TWO_PI))
         ins.append(C.instruction(k, **kw))
     return C.circuit(n, ins)
+
+
+def hea(n: int, depth: int, angles) -> C.Circuit:
+    """HEA circuit; ``angles`` has shape (depth, n, 2): (ry, rz) per qubit."""
+    ins = []
+    for layer in range(depth):
+        for q in range(n):
+            ins.append(C.RyInstruction(qubit=q, theta=float(angles[layer][q][0])))
+        for q in range(n):
+            ins.append(C.RzInstruction(qubit=q, phi=float(angles[layer][q][1])))
+        for q in range(n - 1):
+            ins.append(C.CxInstruction(control=q, target=q + 1))
+        if n > 1:
+            ins.append(C.CxInstruction(control=n - 1, target=0))
+    return C.circuit(n, ins)
+
+
+def hea_batch_arrays(count: int = 4096, seed: int = 42, n_range=(10, 24), d_range=(8, 64)):
+    """(ns, ds, angles) of ``count`` HEA circuits, n ~ U{n_range}, d ~ U{d_range}
+    (inclusive); angles concatenated per instance in (layer, qubit, (ry, rz))
+    order, as ``_core.simulate_hea_batch`` takes them."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    ds = rng.integers(d_range[0], d_range[1] + 1, size=count)
+    angles = [rng.uniform(0.0, TWO_PI, size=(int(d), int(n), 2)) for n, d in zip(ns, ds)]
+    return ns, ds, angles
+
+
+def hea_batch(count: int = 4096, seed: int = 42, n_range=(10, 24), d_range=(8, 64)):
+    """The same circuits as ``hea_batch_arrays`` as circuit objects."""
+    ns, ds, angles = hea_batch_arrays(count, seed, n_range, d_range)
+    return [hea(int(n), int(d), a) for n, d, a in zip(ns, ds, angles)]

@@ -142,9 +142,45 @@ int dq_get_timing(dq_state *s, double *sweep_ms, uint64_t *sweep_launches);
 int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches,
                  uint64_t *gates);
 
+/* Fusion level 3 cost threshold (no reference counterpart; tuning only): a
+ * block of gates is fused into one MFMA GEMM only when its gates' estimated
+ * cost is at least min_cost FP32 flops per amplitude (default 128, about
+ * the cost of the GEMM itself); 0 fuses every block of >= 3 qubits and >= 4
+ * gates. */
+int dq_set_fuse_min_cost(dq_state *s, double min_cost);
+
 /* Fusion level 3: 5-qubit blocks applied on MFMA so far, and the gates they
  * replaced (no reference counterpart). */
 int dq_get_fused_stats(const dq_state *s, uint64_t *blocks, uint64_t *gates);
+
+/* ---- batching: many independent circuits in one call ----
+ * No reference counterpart (the reference simulates one circuit per call;
+ * its closest loop is the per-shot loop of
+ * src/monolithic/statevector/simulator.rs:250-262).  Each instance is a
+ * unitary circuit from |0...0> on its own n qubits plus a diagonal
+ * observable H = sum_t coeff_t Z(zmask_t) (default: the open chain
+ * sum_{i=0}^{n-2} Z_i Z_{i+1}); dq_batch_run writes <psi|H|psi> of every
+ * instance.  Instances whose state fits one workgroup's LDS (n <= 13 for
+ * complex128, n <= 14 for complex64) run entirely in LDS, one workgroup per
+ * instance; larger ones run on the statevector engine from `workers` host
+ * threads, each with its own states and stream. */
+typedef struct dq_batch dq_batch;
+
+int dq_batch_create(int dtype, dq_batch **out);
+void dq_batch_destroy(dq_batch *b);
+/* Appends an instance on n_qubits (1..34); *index receives its number. */
+int dq_batch_add_instance(dq_batch *b, int n_qubits, int *index);
+/* Appends a gate (2^k x 2^k row-major (re, im) matrix, qubits[0] = MSB, as
+ * dq_apply_nq) to an instance. */
+int dq_batch_apply(dq_batch *b, int index, const double *u, const int *qubits, int k);
+int dq_batch_set_observable(dq_batch *b, int index, int n_terms, const uint64_t *zmask,
+                            const double *coeff);
+int dq_batch_set_workers(dq_batch *b, int workers);
+/* Runs every instance; expect[i] = <H> of instance i.  Synchronous. */
+int dq_batch_run(dq_batch *b, double *expect);
+/* Instances on each path in the last run, and the LDS launches' GPU time. */
+int dq_batch_stats(const dq_batch *b, uint64_t *lds_instances, uint64_t *engine_instances,
+                   double *lds_ms);
 
 const char *dq_last_error(void);
 

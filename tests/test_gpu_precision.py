@@ -90,9 +90,10 @@ def random_state(n, rng):
     return psi / np.linalg.norm(psi)
 
 
-def run_c64(n, prog, psi, fuse, tile_bits=0):
+def run_c64(n, prog, psi, fuse, tile_bits=0, min_cost=0.0):
     st = capi.State(n, capi.C64)
     st.set_options(fuse, tile_bits)
+    st.set_fuse_min_cost(min_cost)
     st.set(psi)
     apply(st, prog)
     return st
@@ -144,6 +145,7 @@ def test_fused5_with_measurement_and_outer_ops():
     ref.set(psi)
     st = capi.State(n, capi.C64)
     st.set_options(3, 12)
+    st.set_fuse_min_cost(0.0)
     st.set(psi)
     for rnd in range(3):
         prog = local_program(n, 60, rng, (0, n))
@@ -200,3 +202,21 @@ def test_complex64_renormalisation_keeps_norm():
         st.collapse(q, out, np.sqrt(p1 if out else 1 - p1))
     norm = np.linalg.norm(st.get())
     assert abs(norm - 1.0) <= 1e-5, norm
+
+
+def test_fuse_cost_threshold():
+    """The default cost threshold leaves cheap blocks unfused; a threshold of
+    0 fuses them; both give the oracle's state to FP32 rounding."""
+    n = 13
+    rng = np.random.default_rng(21)
+    psi = random_state(n, rng)
+    prog = local_program(n, 120, rng, (0, 6))
+    want = oracle_run(n, prog, psi)
+    counts = []
+    for cost in (0.0, 128.0, 1e9):
+        st = run_c64(n, prog, psi, fuse=3, tile_bits=12, min_cost=cost)
+        assert np.max(np.abs(st.get() - want)) <= AMP_TOL
+        counts.append(st.fused_stats()[0])
+    assert counts[0] >= counts[1] >= counts[2] == 0 and counts[0] > 0, counts
+    with pytest.raises(capi.DqError):
+        capi.State(8, capi.C64).set_fuse_min_cost(-1.0)
