@@ -24,7 +24,7 @@ EXPORTS = (
     "dq_copy_to_host", "dq_synchronize", "dq_state_copy", "dq_apply_pauli_sum", "dq_pauli_inner",
     "dq_compare_states",
     "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
-    "dq_get_fused_stats", "dq_set_fuse_min_cost",
+    "dq_get_fused_stats", "dq_set_fuse_min_cost", "dq_set_pair_fusion",
     "dq_batch_create", "dq_batch_destroy", "dq_batch_add_instance", "dq_batch_apply",
     "dq_batch_set_observable", "dq_batch_set_workers", "dq_batch_run", "dq_batch_stats",
     "dq_last_error",
@@ -61,6 +61,7 @@ def lib() -> ctypes.CDLL:
         L.dq_get_stats.argtypes = [vp, u64p, u64p, u64p]
         L.dq_get_fused_stats.argtypes = [vp, u64p, u64p]
         L.dq_set_fuse_min_cost.argtypes = [vp, ctypes.c_double]
+        L.dq_set_pair_fusion.argtypes = [vp, ctypes.c_int]
         L.dq_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
         L.dq_batch_destroy.argtypes = [vp]
         L.dq_batch_destroy.restype = None
@@ -198,6 +199,9 @@ class State:
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().dq_get_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def set_pair_fusion(self, on: bool) -> None:
+        _check(lib().dq_set_pair_fusion(self.h, int(bool(on))))
 
     def set_fuse_min_cost(self, min_cost: float) -> None:
         _check(lib().dq_set_fuse_min_cost(self.h, float(min_cost)))

@@ -494,6 +494,7 @@ struct dq_state {
     uint64_t sweeps = 0, launches = 0, gates = 0;
     uint64_t fused_blocks = 0, fused_gates = 0;  // fusion level 3 statistics
     double fuse_min_cost = 128.0;                // fusion level 3 cost threshold
+    int pair_fusion = 1;                         // fusion levels 2-3: merge gates on <= 2 qubits
     // optional per-launch timing of the sweep kernels (HIP events on `stream`)
     int timing = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
@@ -514,6 +515,7 @@ PlanConfig plan_config(const dq_state *s) {
     c.low_bits = s->low_bits;
     c.reg_tiles = s->reg_tiles;
     c.fuse_min_cost = s->fuse_min_cost;
+    c.pair_fusion = s->pair_fusion;
     return c;
 }
 
@@ -1436,6 +1438,14 @@ int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches, uint64
     if (sweeps) *sweeps = s->sweeps;
     if (launches) *launches = s->launches;
     if (gates) *gates = s->gates;
+    return DQ_OK;
+}
+
+int dq_set_pair_fusion(dq_state *s, int on) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (on != 0 && on != 1) return fail(DQ_ERR_ARG, "pair fusion flag must be 0 or 1");
+    if (int rc = flush(s)) return rc;
+    s->pair_fusion = on;
     return DQ_OK;
 }
 

@@ -25,6 +25,7 @@ struct PlanConfig {
     int low_bits = 5;    // qubits 0..low_bits-1 are in every tile (coalesced runs)
     int reg_tiles = 1;   // register-resident tiles for tiles of >= 9 qubits
     double fuse_min_cost = 128.0;  // fusion level 3: least cost of a fused block's gates
+    int pair_fusion = 1;           // fusion levels 2-3: merge neighbouring gates on <= 2 qubits
 };
 
 inline bool dq_tile_bits_supported(int b) { return b == 9 || b == 12 || b == 13; }
@@ -286,10 +287,9 @@ inline double gate_cost(const HostOp &h) {
 // kFuseMinQubits qubits and kFuseMinOps gates, whose gates cost at least
 // min_cost, become one K_FUSED op (their FP64 product, stored in
 // `store`); other blocks keep their gates.
-inline std::vector<const HostOp *> fuse_blocks(const std::vector<HostOp> &queue, const Pass &p, const int *pos,
-                                               const int *tq, int b, double min_cost, std::deque<HostOp> &store) {
-    std::vector<const HostOp *> out, rest, next;
-    for (int idx : p.ops) rest.push_back(&queue[idx]);
+inline std::vector<const HostOp *> fuse_blocks(const std::vector<const HostOp *> &in, const int *pos, const int *tq,
+                                               int b, double min_cost, std::deque<HostOp> &store) {
+    std::vector<const HostOp *> out, rest(in), next;
     auto tile_op = [&](const HostOp &h) {
         if (h.kind == K_COLLAPSE || h.k > 5) return false;
         for (int j = 0; j < h.k; ++j)
@@ -342,6 +342,114 @@ inline std::vector<const HostOp *> fuse_blocks(const std::vector<HostOp> &queue,
             out.insert(out.end(), blk.begin(), blk.end());
         }
         rest.swap(next);
+    }
+    return out;
+}
+
+// Pair fusion (fusion levels 2 and 3): gates on at most 2 tile qubits that
+// are neighbours on every qubit's timeline merge into one op, their FP64
+// product, classified again (a product of diagonals stays diagonal).  Walk
+// the ops in order keeping, per qubit, the op that last touched it (if it is
+// a mergeable op of <= 2 tile qubits):
+//   * a 1-qubit gate on a joins the last op on a;
+//   * a 2-qubit gate on (a, b) joins the last op on a if that op is the last
+//     op on b too (same pair); else it absorbs the last ops on a and b when
+//     those act on a (resp. b) alone;
+//   * any other op (3+ qubits, outer qubits, collapse) is a barrier on its
+//     qubits.
+// A merged gate only moves past ops that touch none of its qubits (the op it
+// joins is the last one on each of them), so the product is the same
+// operator as the sequence.
+inline std::vector<const HostOp *> fuse_pairs(const std::vector<const HostOp *> &in, const int *pos,
+                                              std::deque<HostOp> &store) {
+    struct FOp {
+        std::vector<const HostOp *> gates;
+        uint64_t qm = 0;
+        bool dead = false;
+    };
+    std::vector<FOp> ops;
+    ops.reserve(in.size());
+    int last[64];
+    for (int q = 0; q < 64; ++q) last[q] = -1;
+    auto mergeable = [&](const HostOp &h) {
+        if (h.kind == K_COLLAPSE || h.k > 2) return false;
+        for (int j = 0; j < h.k; ++j)
+            if (pos[h.q[j]] >= kOuterBit) return false;
+        return true;
+    };
+    for (const HostOp *hp : in) {
+        const HostOp &h = *hp;
+        const uint64_t qm = h.qmask();
+        if (!mergeable(h)) {
+            FOp f;
+            f.gates.push_back(hp);
+            f.qm = qm;
+            ops.push_back(f);
+            for (int j = 0; j < h.k; ++j) last[h.q[j]] = -1;
+            continue;
+        }
+        if (h.k == 1) {
+            const int a = h.q[0];
+            if (last[a] >= 0) {
+                ops[size_t(last[a])].gates.push_back(hp);
+                continue;
+            }
+            FOp f;
+            f.gates.push_back(hp);
+            f.qm = qm;
+            ops.push_back(f);
+            last[a] = int(ops.size() - 1);
+            continue;
+        }
+        const int a = h.q[0], bq = h.q[1];
+        const int la = last[a], lb = last[bq];
+        if (la >= 0 && la == lb) {  // the last op on both is on exactly this pair
+            ops[size_t(la)].gates.push_back(hp);
+            continue;
+        }
+        // a new op at this position absorbs the single-qubit predecessors on a
+        // and b (each is the last op on its qubit, so it moves forward past
+        // ops that do not touch that qubit)
+        FOp f;
+        for (const int l : {la, lb}) {
+            if (l < 0) continue;
+            FOp &P = ops[size_t(l)];
+            if (P.qm != (uint64_t(1) << a) && P.qm != (uint64_t(1) << bq)) continue;
+            f.gates.insert(f.gates.end(), P.gates.begin(), P.gates.end());
+            P.dead = true;
+        }
+        f.gates.push_back(hp);
+        f.qm = qm;
+        ops.push_back(f);
+        last[a] = last[bq] = int(ops.size() - 1);
+    }
+    std::vector<const HostOp *> out;
+    out.reserve(ops.size());
+    for (const FOp &f : ops) {
+        if (f.dead) continue;
+        if (f.gates.size() == 1) {
+            out.push_back(f.gates[0]);
+            continue;
+        }
+        std::vector<int> qs;
+        for (const HostOp *g : f.gates)
+            for (int j = 0; j < g->k; ++j)
+                if (std::find(qs.begin(), qs.end(), g->q[j]) == qs.end()) qs.push_back(g->q[j]);
+        const int m = int(qs.size()), dim = 1 << m;
+        std::vector<cd> M(size_t(dim) * dim, cd(0, 0));
+        for (int r = 0; r < dim; ++r) M[size_t(r) * dim + r] = cd(1, 0);
+        for (const HostOp *g : f.gates) embed_apply(M, qs, *g);
+        std::vector<double> flat(2 * M.size());
+        for (size_t e = 0; e < M.size(); ++e) {
+            flat[2 * e] = M[e].real();
+            flat[2 * e + 1] = M[e].imag();
+        }
+        HostOp fused;
+        fused.k = m;
+        for (int j = 0; j < m; ++j) fused.q[j] = qs[size_t(j)];
+        classify(fused, flat.data());
+        store.push_back(std::move(fused));
+        out.push_back(&store.back());
     }
     return out;
 }
@@ -429,16 +537,16 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     // the low qubits stay thread bits at load/store (coalesced runs)
     const unsigned lowmask = (1u << std::max(0, std::min(s->low_bits, b - 4))) - 1;
     const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - 4)) - 1);
-    std::deque<HostOp> fused_store;
+    std::deque<HostOp> fused_store, pair_store;
     std::vector<const HostOp *> eff;
+    for (int idx : p.ops) eff.push_back(&queue[idx]);
+    if (s->fuse >= 2 && s->pair_fusion) eff = fuse_pairs(eff, pos, pair_store);
     if (s->fuse == 3 && b >= 12) {
-        eff = fuse_blocks(queue, p, pos, pd.tq, b, s->fuse_min_cost, fused_store);
+        eff = fuse_blocks(eff, pos, pd.tq, b, s->fuse_min_cost, fused_store);
         for (const HostOp &f : fused_store) {
             ++out.fused_blocks;
             out.fused_gates += f.outcome;
         }
-    } else {
-        for (int idx : p.ops) eff.push_back(&queue[idx]);
     }
     const size_t nops = eff.size();
     std::vector<unsigned> hard(nops, 0), soft(nops, 0);

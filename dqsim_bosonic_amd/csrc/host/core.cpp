@@ -443,6 +443,7 @@ class Engine {
         return py::make_tuple(a, b, c);
     }
     void set_fuse_min_cost(double c) { check(dq_set_fuse_min_cost(st_->s, c), "fusion cost threshold"); }
+    void set_pair_fusion(bool on) { check(dq_set_pair_fusion(st_->s, on ? 1 : 0), "pair fusion"); }
     py::tuple fused_stats() {
         uint64_t a = 0, b = 0;
         check(dq_get_fused_stats(st_->s, &a, &b), "fused stats");
@@ -911,6 +912,7 @@ PYBIND11_MODULE(_core, m) {
         .def("stats", &Engine::stats)
         .def("fused_stats", &Engine::fused_stats)
         .def("set_fuse_min_cost", &Engine::set_fuse_min_cost)
+        .def("set_pair_fusion", &Engine::set_pair_fusion)
         .def("compare", &Engine::compare)
         .def("result", &Engine::result);
 
@@ -972,7 +974,7 @@ PYBIND11_MODULE(_core, m) {
     // qubits[-1] under controls qubits[:-1] (as dq_apply_1q).  Returns, in
     // order, (qubits, 32x32 matrix, gate count) for a fused block or
     // (original index,) for an op kept as it is.
-    m.def("_fuse_blocks", [](std::vector<int> tile_qubits, py::list ops, double min_cost) {
+    m.def("_fuse_blocks", [](std::vector<int> tile_qubits, py::list ops, double min_cost, int max_qubits) {
         std::vector<dq::HostOp> queue;
         for (py::handle item : ops) {
             py::tuple t = py::reinterpret_borrow<py::tuple>(item);
@@ -1006,21 +1008,24 @@ PYBIND11_MODULE(_core, m) {
             pos[tile_qubits[t]] = int(t);
             tq[t] = tile_qubits[t];
         }
-        dq::Pass p;
-        for (size_t i = 0; i < queue.size(); ++i) p.ops.push_back(int(i));
-        std::deque<dq::HostOp> store;
-        auto eff = dq::fuse_blocks(queue, p, pos, tq, int(tile_qubits.size()), min_cost, store);
+        std::vector<const dq::HostOp *> in;
+        for (const dq::HostOp &h : queue) in.push_back(&h);
+        std::deque<dq::HostOp> store, pstore;
+        auto eff = max_qubits == 2 ? dq::fuse_pairs(in, pos, pstore)
+                                   : dq::fuse_blocks(in, pos, tq, int(tile_qubits.size()), min_cost, store);
         py::list out;
         for (const dq::HostOp *h : eff) {
-            if (h->kind == dq::K_FUSED) {
-                py::array_t<std::complex<double>> m({32, 32});
-                std::memcpy(m.mutable_data(), h->dense.data(), h->dense.size() * sizeof(double));
-                out.append(py::make_tuple(std::vector<int>(h->q, h->q + 5), m, h->outcome));
-            } else {
+            if (h >= queue.data() && h < queue.data() + queue.size()) {
                 out.append(py::make_tuple(int(h - queue.data())));
+                continue;
             }
+            const size_t d = size_t(1) << h->k;
+            py::array_t<std::complex<double>> m({d, d});
+            std::memcpy(m.mutable_data(), h->dense.data(), h->dense.size() * sizeof(double));
+            std::vector<int> qs(h->q, h->q + h->k);
+            out.append(py::make_tuple(qs, m, h->kind == dq::K_FUSED ? h->outcome : -1));
         }
         return out;
-    }, py::arg("tile_qubits"), py::arg("ops"), py::arg("min_cost") = 0.0);
+    }, py::arg("tile_qubits"), py::arg("ops"), py::arg("min_cost") = 0.0, py::arg("max_qubits") = 5);
     m.def("_parse_mode", [](const std::string &mode) { return int(parse_mode(mode)); });
 }

@@ -142,6 +142,11 @@ int dq_get_timing(dq_state *s, double *sweep_ms, uint64_t *sweep_launches);
 int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches,
                  uint64_t *gates);
 
+/* Pair fusion at fusion levels 2 and 3 (no reference counterpart; tuning
+ * only; default on): neighbouring gates on at most 2 tile qubits are merged
+ * into one op (their FP64 product, classified again). */
+int dq_set_pair_fusion(dq_state *s, int on);
+
 /* Fusion level 3 cost threshold (no reference counterpart; tuning only): a
  * block of gates is fused into one MFMA GEMM only when its gates' estimated
  * cost is at least min_cost FP32 flops per amplitude (default 128, about

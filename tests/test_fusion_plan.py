@@ -1,8 +1,9 @@
-"""CPU tests of fusion level 3's block former (no device needed).
+"""CPU tests of the planner's gate fusion (no device needed).
 
-The planner groups gates into blocks on at most 5 qubits and multiplies each
-block into one 32x32 matrix in FP64; gates it overtakes must commute with the
-block.  The check is the mathematics: applying the rewritten list (fused
+Fusion level 3 groups gates into blocks on at most 5 qubits and multiplies
+each block into one 32x32 matrix in FP64; pair fusion (levels 2 and 3)
+merges neighbouring gates on at most 2 qubits.  Gates moved past others
+must commute with them.  The check is the mathematics: applying the rewritten list (fused
 blocks plus the gates left as they are) to a random state must give the same
 state as the original gate sequence, to FP64 rounding, computed with the
 oracle's gate application.  A wrong embedding order, a reordering across a
@@ -102,3 +103,48 @@ def test_blocks_form_on_layered_circuit():
     fused = sum(item[2] for item in plan if len(item) == 3)
     blocks = sum(1 for item in plan if len(item) == 3)
     assert blocks > 0 and fused >= 0.8 * len(ops), (blocks, fused, len(ops))
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("n,tile", [(8, list(range(8))), (9, [0, 1, 2, 3, 5, 7, 8]), (7, [6, 5, 4, 3, 2, 1, 0])])
+def test_pair_fused_list_equals_program(seed, n, tile):
+    rng = np.random.default_rng(100 + seed)
+    ops = random_ops(n, 80, rng)
+    plan = core._fuse_blocks(tile, [(qs, u.ravel(), cv) for qs, u, cv in ops], max_qubits=2)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    want = psi.copy()
+    for qs, u, cv in ops:
+        want = apply_op(want, n, qs, u, cv)
+    got = psi.copy()
+    seen = []
+    for item in plan:
+        if len(item) == 1:
+            qs, u, cv = ops[item[0]]
+            got = apply_op(got, n, qs, u, cv)
+            seen.append(item[0])
+        else:
+            qs, m, _ = item
+            assert len(qs) <= 2 and all(q in tile for q in qs)
+            assert np.allclose(m.conj().T @ m, np.eye(m.shape[0]), atol=1e-12)
+            if len(qs) == 1:
+                orc.apply_one_qubit(got, n, m, qs[0])
+            else:
+                orc.apply_n_qubit(got, n, m, qs)
+    assert len(set(seen)) == len(seen)
+    assert len(plan) < len(ops)
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def test_pair_fusion_on_layered_circuit():
+    """1q layers + cz pairs: each pair's gates become one op."""
+    n = 10
+    rng = np.random.default_rng(2)
+    ops = []
+    for layer in range(4):
+        for q in range(n):
+            ops.append(([q], orc.gate_matrix("u3", rng.uniform(-4, 4, 3)).ravel(), []))
+        for q in range(layer % 2, n - 1, 2):
+            ops.append(([q, q + 1], orc.gate_matrix("cz", []).ravel(), []))
+    plan = core._fuse_blocks(list(range(n)), ops, max_qubits=2)
+    assert len(plan) <= len(ops) // 2, (len(plan), len(ops))

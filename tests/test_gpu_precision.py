@@ -90,10 +90,13 @@ def random_state(n, rng):
     return psi / np.linalg.norm(psi)
 
 
-def run_c64(n, prog, psi, fuse, tile_bits=0, min_cost=0.0):
+def run_c64(n, prog, psi, fuse, tile_bits=0, min_cost=0.0, pair=False):
+    """pair=False: blocks are formed from the gates as given (pair fusion,
+    on by default, would first merge neighbouring gates)."""
     st = capi.State(n, capi.C64)
     st.set_options(fuse, tile_bits)
     st.set_fuse_min_cost(min_cost)
+    st.set_pair_fusion(pair)
     st.set(psi)
     apply(st, prog)
     return st
@@ -146,6 +149,7 @@ def test_fused5_with_measurement_and_outer_ops():
     st = capi.State(n, capi.C64)
     st.set_options(3, 12)
     st.set_fuse_min_cost(0.0)
+    st.set_pair_fusion(False)
     st.set(psi)
     for rnd in range(3):
         prog = local_program(n, 60, rng, (0, n))
@@ -220,3 +224,15 @@ def test_fuse_cost_threshold():
     assert counts[0] >= counts[1] >= counts[2] == 0 and counts[0] > 0, counts
     with pytest.raises(capi.DqError):
         capi.State(8, capi.C64).set_fuse_min_cost(-1.0)
+
+
+@pytest.mark.parametrize("fuse", [2, 3])
+def test_pair_fusion_then_blocks_match_oracle(fuse):
+    """Default planning (pair fusion on) at fusion levels 2 and 3."""
+    n = 14
+    rng = np.random.default_rng(31)
+    psi = random_state(n, rng)
+    prog = local_program(n, 200, rng, (0, 9))
+    want = oracle_run(n, prog, psi)
+    st = run_c64(n, prog, psi, fuse=fuse, tile_bits=12, min_cost=0.0, pair=True)
+    assert np.max(np.abs(st.get() - want)) <= AMP_TOL
