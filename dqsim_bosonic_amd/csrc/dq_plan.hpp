@@ -40,6 +40,8 @@ struct HostOp {
     std::vector<double> mat;    // (re, im) pairs: 4 (2x2), 2^k (diag/mono), 4^k (dense)
     std::vector<int> perm;      // K_MONO
     std::vector<double> dense;  // the full 2^k x 2^k matrix as given (register/LDS paths)
+    int nsel = 0;               // > 0: one matrix per value of these outer qubits (pair fusion)
+    int selq[3] = {0, 0, 0};    // selector qubits; selector bit j = qubit selq[j]
     uint64_t qmask() const {
         uint64_t m = 0;
         for (int j = 0; j < k; ++j) m |= uint64_t(1) << q[j];
@@ -347,81 +349,105 @@ inline std::vector<const HostOp *> fuse_blocks(const std::vector<const HostOp *>
 }
 
 // Pair fusion (fusion levels 2 and 3): gates on at most 2 tile qubits that
-// are neighbours on every qubit's timeline merge into one op, their FP64
+// are neighbours on every tile qubit's timeline merge into one op, their FP64
 // product, classified again (a product of diagonals stays diagonal).  Walk
-// the ops in order keeping, per qubit, the op that last touched it (if it is
-// a mergeable op of <= 2 tile qubits):
+// the ops in order keeping, per tile qubit, the op that last touched it (if
+// it is a mergeable op):
 //   * a 1-qubit gate on a joins the last op on a;
 //   * a 2-qubit gate on (a, b) joins the last op on a if that op is the last
 //     op on b too (same pair); else it absorbs the last ops on a and b when
 //     those act on a (resp. b) alone;
-//   * any other op (3+ qubits, outer qubits, collapse) is a barrier on its
-//     qubits.
-// A merged gate only moves past ops that touch none of its qubits (the op it
-// joins is the last one on each of them), so the product is the same
+//   * any other op (3+ tile qubits, collapse) is a barrier on its qubits.
+// A merged gate only moves past ops that touch none of its tile qubits (the
+// op it joins is the last one on each of them), so the product is the same
 // operator as the sequence.
+// Gates may also act on qubits outside the tile if they are diagonal in them
+// (diagonal gates, controls of a controlled 2x2): every op of a tile pass
+// touches outer qubits only diagonally, so such qubits never order ops.  The
+// merged op then holds one matrix per value of its (at most kMaxSel) outer
+// qubits, which are constant over a tile: the sweep picks the matrix per tile.
+constexpr int kMaxSel = 3;
+
 inline std::vector<const HostOp *> fuse_pairs(const std::vector<const HostOp *> &in, const int *pos,
                                               std::deque<HostOp> &store) {
     struct FOp {
         std::vector<const HostOp *> gates;
-        uint64_t qm = 0;
+        uint64_t qm = 0;  // tile qubits
+        uint64_t om = 0;  // outer (selector) qubits
         bool dead = false;
     };
     std::vector<FOp> ops;
     ops.reserve(in.size());
     int last[64];
     for (int q = 0; q < 64; ++q) last[q] = -1;
-    auto mergeable = [&](const HostOp &h) {
-        if (h.kind == K_COLLAPSE || h.k > 2) return false;
-        for (int j = 0; j < h.k; ++j)
-            if (pos[h.q[j]] >= kOuterBit) return false;
-        return true;
+    auto split = [&](const HostOp &h, uint64_t &tm, uint64_t &om) {
+        tm = om = 0;
+        for (int j = 0; j < h.k; ++j) (pos[h.q[j]] < kOuterBit ? tm : om) |= uint64_t(1) << h.q[j];
+    };
+    auto mergeable = [&](const HostOp &h, uint64_t tm, uint64_t om) {
+        if (h.kind == K_COLLAPSE) return false;
+        const int nt = __builtin_popcountll(tm);
+        if (nt < 1 || nt > 2 || __builtin_popcountll(om) > kMaxSel) return false;
+        if (!om) return true;
+        if (h.kind == K_DIAG) return true;
+        // a controlled 2x2 with every outer qubit among its controls
+        return h.kind == K_CTRL1 && pos[h.q[h.k - 1]] < kOuterBit;
     };
     for (const HostOp *hp : in) {
         const HostOp &h = *hp;
-        const uint64_t qm = h.qmask();
-        if (!mergeable(h)) {
+        uint64_t tm, om;
+        split(h, tm, om);
+        if (!mergeable(h, tm, om)) {
             FOp f;
             f.gates.push_back(hp);
-            f.qm = qm;
+            f.qm = tm;
+            f.om = om;
             ops.push_back(f);
             for (int j = 0; j < h.k; ++j) last[h.q[j]] = -1;
             continue;
         }
-        if (h.k == 1) {
-            const int a = h.q[0];
-            if (last[a] >= 0) {
-                ops[size_t(last[a])].gates.push_back(hp);
+        const int a = __builtin_ctzll(tm);
+        const int b = __builtin_popcountll(tm) == 2 ? 63 - __builtin_clzll(tm) : -1;
+        auto fits = [&](const FOp &f) { return __builtin_popcountll(f.om | om) <= kMaxSel; };
+        if (b < 0) {
+            if (last[a] >= 0 && fits(ops[size_t(last[a])])) {
+                FOp &F = ops[size_t(last[a])];
+                F.gates.push_back(hp);
+                F.om |= om;
                 continue;
             }
             FOp f;
             f.gates.push_back(hp);
-            f.qm = qm;
+            f.qm = tm;
+            f.om = om;
             ops.push_back(f);
             last[a] = int(ops.size() - 1);
             continue;
         }
-        const int a = h.q[0], bq = h.q[1];
-        const int la = last[a], lb = last[bq];
-        if (la >= 0 && la == lb) {  // the last op on both is on exactly this pair
+        const int la = last[a], lb = last[b];
+        if (la >= 0 && la == lb && fits(ops[size_t(la)])) {  // the last op on both is on exactly this pair
             ops[size_t(la)].gates.push_back(hp);
+            ops[size_t(la)].om |= om;
             continue;
         }
         // a new op at this position absorbs the single-qubit predecessors on a
         // and b (each is the last op on its qubit, so it moves forward past
         // ops that do not touch that qubit)
         FOp f;
+        f.om = om;
         for (const int l : {la, lb}) {
             if (l < 0) continue;
             FOp &P = ops[size_t(l)];
-            if (P.qm != (uint64_t(1) << a) && P.qm != (uint64_t(1) << bq)) continue;
+            if (P.dead || (P.qm != (uint64_t(1) << a) && P.qm != (uint64_t(1) << b))) continue;
+            if (__builtin_popcountll(f.om | P.om) > kMaxSel) continue;
             f.gates.insert(f.gates.end(), P.gates.begin(), P.gates.end());
+            f.om |= P.om;
             P.dead = true;
         }
         f.gates.push_back(hp);
-        f.qm = qm;
+        f.qm = tm;
         ops.push_back(f);
-        last[a] = last[bq] = int(ops.size() - 1);
+        last[a] = last[b] = int(ops.size() - 1);
     }
     std::vector<const HostOp *> out;
     out.reserve(ops.size());
@@ -431,10 +457,16 @@ inline std::vector<const HostOp *> fuse_pairs(const std::vector<const HostOp *> 
             out.push_back(f.gates[0]);
             continue;
         }
-        std::vector<int> qs;
+        // tile qubits in order of first appearance, then the outer qubits
+        std::vector<int> tqs, oqs;
         for (const HostOp *g : f.gates)
-            for (int j = 0; j < g->k; ++j)
-                if (std::find(qs.begin(), qs.end(), g->q[j]) == qs.end()) qs.push_back(g->q[j]);
+            for (int j = 0; j < g->k; ++j) {
+                const int q = g->q[j];
+                auto &v = pos[q] < kOuterBit ? tqs : oqs;
+                if (std::find(v.begin(), v.end(), q) == v.end()) v.push_back(q);
+            }
+        std::vector<int> qs(tqs);
+        qs.insert(qs.end(), oqs.begin(), oqs.end());
         const int m = int(qs.size()), dim = 1 << m;
         std::vector<cd> M(size_t(dim) * dim, cd(0, 0));
         for (int r = 0; r < dim; ++r) M[size_t(r) * dim + r] = cd(1, 0);
@@ -448,7 +480,36 @@ inline std::vector<const HostOp *> fuse_pairs(const std::vector<const HostOp *> 
         fused.k = m;
         for (int j = 0; j < m; ++j) fused.q[j] = qs[size_t(j)];
         classify(fused, flat.data());
-        store.push_back(std::move(fused));
+        if (!oqs.empty() && !(fused.kind == K_DIAG && m <= 2)) {
+            // one 2^kt x 2^kt matrix per value of the outer qubits (the
+            // product is block-diagonal in them): rows and columns of M whose
+            // outer bits (the low ones, after the tile qubits) equal s
+            const int kt = int(tqs.size()), no = int(oqs.size()), sd = 1 << kt;
+            HostOp sel;
+            sel.kind = kt == 1 ? K_DENSE1 : K_DENSEK;
+            sel.k = kt;
+            for (int j = 0; j < kt; ++j) sel.q[j] = tqs[size_t(j)];
+            sel.nsel = no;
+            for (int j = 0; j < no; ++j) sel.selq[j] = oqs[size_t(j)];
+            sel.dense.resize(size_t(2) * (size_t(1) << no) * sd * sd);
+            size_t w = 0;
+            for (int sv = 0; sv < (1 << no); ++sv) {
+                // selector value bit j = outer qubit oqs[j], which is index bit no-1-j of M's low part
+                int low = 0;
+                for (int j = 0; j < no; ++j)
+                    if ((sv >> j) & 1) low |= 1 << (no - 1 - j);
+                for (int r = 0; r < sd; ++r)
+                    for (int c = 0; c < sd; ++c) {
+                        const cd v = M[size_t((r << no) | low) * dim + size_t((c << no) | low)];
+                        sel.dense[w++] = v.real();
+                        sel.dense[w++] = v.imag();
+                    }
+            }
+            sel.mat = sel.dense;
+            store.push_back(std::move(sel));
+        } else {
+            store.push_back(std::move(fused));
+        }
         out.push_back(&store.back());
     }
     return out;
@@ -477,13 +538,12 @@ inline int rb_slot(int packed, int tile_bit) {
 }
 
 // Variant of a 2x2 or 4x4 register op (see OpVariant).
-inline int op_variant(const std::vector<double> &m) {
+inline int op_variant(const std::vector<double> &m, int dim) {
     const size_t cnt = m.size() / 2;
-    const int dim = cnt == 4 ? 2 : 4;
     bool real = true, diag = true, anti = true;
     for (size_t e = 0; e < cnt; ++e) {
         const double re = m[2 * e], im = m[2 * e + 1];
-        const int r = int(e) / dim, c = int(e) % dim;
+        const int r = int(e % size_t(dim * dim)) / dim, c = int(e) % dim;  // several matrices: per matrix
         const bool zero = re == 0.0 && im == 0.0;
         if (im != 0.0) real = false;
         if (r != c && !zero) diag = false;
@@ -699,8 +759,12 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
         }
         t.mat = int(mats.size() / 2);
         mats.insert(mats.end(), m->begin(), m->end());
-        if (t.kind == T_DENSE1 || t.kind == T_DENSE2) t.variant = op_variant(*m);
-        if (t.kind == T_CTRL1) t.variant = op_variant(*m) == V_ANTI ? V_ANTI : V_GEN;
+        if (t.kind == T_DENSE1 || t.kind == T_DENSE2) t.variant = op_variant(*m, t.kind == T_DENSE1 ? 2 : 4);
+        if (t.kind == T_CTRL1) t.variant = op_variant(*m, 2) == V_ANTI ? V_ANTI : V_GEN;
+        if ((t.kind == T_DENSE1 || t.kind == T_DENSE2) && h.nsel > 0) {
+            t.nsel = h.nsel;
+            for (int j = 0; j < h.nsel; ++j) t.sel[j] = pos[h.selq[j]] - kOuterBit;
+        }
         out.ops.push_back(t);
     }
 }

@@ -411,6 +411,11 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 cur = op.rb;
             }
             const cplx<T> *m = mats + op.mat;
+            if (op.nsel > 0) {  // one matrix per value of the selector bits (constant over the tile)
+                unsigned e = 0;
+                for (int j = 0; j < op.nsel; ++j) e |= unsigned((tile >> op.sel[j]) & 1) << j;
+                m += e * (op.kind == T_DENSE2 ? 16u : 4u);
+            }
             switch (op.kind) {
             case T_DENSE1:
                 dispatch_dense1<T>(a, op.p, op.variant, m);

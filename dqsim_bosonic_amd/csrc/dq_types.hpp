@@ -61,6 +61,8 @@ struct TileOp {
     int outcome;
     int variant;
     unsigned ocmask, ocval;  // T_CTRL1 controls on outer qubits (outer-index bits)
+    int nsel;      // T_DENSE1/T_DENSE2: > 0 selects matrix number sum_j bit(sel[j]) << j
+    int sel[3];    // outer-index bits of the selector
     double norm;
 };
 

@@ -1019,11 +1019,11 @@ PYBIND11_MODULE(_core, m) {
                 out.append(py::make_tuple(int(h - queue.data())));
                 continue;
             }
-            const size_t d = size_t(1) << h->k;
-            py::array_t<std::complex<double>> m({d, d});
+            const size_t d = size_t(1) << h->k, ns = size_t(1) << h->nsel;
+            py::array_t<std::complex<double>> m({ns, d, d});
             std::memcpy(m.mutable_data(), h->dense.data(), h->dense.size() * sizeof(double));
-            std::vector<int> qs(h->q, h->q + h->k);
-            out.append(py::make_tuple(qs, m, h->kind == dq::K_FUSED ? h->outcome : -1));
+            std::vector<int> qs(h->q, h->q + h->k), sq(h->selq, h->selq + h->nsel);
+            out.append(py::make_tuple(qs, m, h->kind == dq::K_FUSED ? h->outcome : -1, sq));
         }
         return out;
     }, py::arg("tile_qubits"), py::arg("ops"), py::arg("min_cost") = 0.0, py::arg("max_qubits") = 5);

@@ -78,7 +78,8 @@ def test_fused_list_equals_program(seed, n, tile):
             got = apply_op(got, n, qs, u, cv)
             seen.append(item[0])
         else:
-            qs, m, cnt = item
+            qs, m, cnt, _ = item
+            m = m[0]
             assert len(set(qs)) == 5 and all(q in tile for q in qs)
             assert np.allclose(m.conj().T @ m, np.eye(32), atol=1e-12)
             orc.apply_n_qubit(got, n, m, qs)
@@ -100,16 +101,64 @@ def test_blocks_form_on_layered_circuit():
         for q in range(layer % 2, n - 1, 2):
             ops.append(([q, q + 1], orc.gate_matrix("cz", []).ravel(), []))
     plan = core._fuse_blocks(list(range(n)), ops)
-    fused = sum(item[2] for item in plan if len(item) == 3)
-    blocks = sum(1 for item in plan if len(item) == 3)
+    fused = sum(item[2] for item in plan if len(item) == 4)
+    blocks = sum(1 for item in plan if len(item) == 4)
     assert blocks > 0 and fused >= 0.8 * len(ops), (blocks, fused, len(ops))
 
 
+def pass_ops(n, tile, count, rng):
+    """Ops as a tile pass holds them: non-diagonal action only on tile
+    qubits; qubits outside the tile appear only in diagonal gates and as
+    controls (with control values) of controlled 2x2 gates."""
+    outer = [q for q in range(n) if q not in tile]
+    ops = []
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.3:
+            g = GATES_1Q[int(rng.integers(len(GATES_1Q)))]
+            ops.append(([int(rng.choice(tile))], orc.gate_matrix(g, rng.uniform(-4, 4, NPARAM.get(g, 0))), []))
+        elif r < 0.5:
+            g = GATES_2Q[int(rng.integers(len(GATES_2Q)))]
+            qs = [int(x) for x in rng.choice(tile, size=2, replace=False)]
+            ops.append((qs, orc.gate_matrix(g, rng.uniform(-4, 4, NPARAM.get(g, 0))), []))
+        elif r < 0.75:  # diagonal gate on any qubits
+            g = ["cz", "cp", "rzz", "t", "rz", "s"][int(rng.integers(6))]
+            u = orc.gate_matrix(g, rng.uniform(-4, 4, NPARAM.get(g, 0)))
+            k = int(np.log2(u.shape[0]))
+            ops.append(([int(x) for x in rng.choice(n, size=k, replace=False)], u, []))
+        elif r < 0.9 and outer:  # controlled 2x2, target in the tile, controls anywhere
+            t = int(rng.choice(tile))
+            cs = [int(x) for x in rng.choice([q for q in range(n) if q != t], size=2, replace=False)]
+            ops.append((cs + [t], orc.gate_matrix("u3", rng.uniform(-4, 4, 3)),
+                        [int(rng.integers(2)), int(rng.integers(2))]))
+        else:
+            qs = [int(x) for x in rng.choice(tile, size=3, replace=False)]
+            ops.append((qs, orc.gate_matrix("ccx", []), []))
+    return ops
+
+
+def apply_selected(psi, n, qs, mats, sq):
+    """One matrix per value of the selector qubits sq (bit j = qubit sq[j])."""
+    idx = np.arange(1 << n)
+    out = psi.copy()
+    for sv in range(mats.shape[0]):
+        mask = np.ones(1 << n, dtype=bool)
+        for j, q in enumerate(sq):
+            mask &= ((idx >> q) & 1) == ((sv >> j) & 1)
+        sub = psi.copy()
+        if len(qs) == 1:
+            orc.apply_one_qubit(sub, n, mats[sv], qs[0])
+        else:
+            orc.apply_n_qubit(sub, n, mats[sv], qs)
+        out = np.where(mask, sub, out)
+    return out
+
+
 @pytest.mark.parametrize("seed", range(10))
-@pytest.mark.parametrize("n,tile", [(8, list(range(8))), (9, [0, 1, 2, 3, 5, 7, 8]), (7, [6, 5, 4, 3, 2, 1, 0])])
+@pytest.mark.parametrize("n,tile", [(8, list(range(8))), (9, [0, 1, 2, 3, 5, 7, 8]), (10, [9, 8, 7, 6, 2, 1])])
 def test_pair_fused_list_equals_program(seed, n, tile):
     rng = np.random.default_rng(100 + seed)
-    ops = random_ops(n, 80, rng)
+    ops = pass_ops(n, tile, 80, rng)
     plan = core._fuse_blocks(tile, [(qs, u.ravel(), cv) for qs, u, cv in ops], max_qubits=2)
     psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
     psi /= np.linalg.norm(psi)
@@ -124,13 +173,13 @@ def test_pair_fused_list_equals_program(seed, n, tile):
             got = apply_op(got, n, qs, u, cv)
             seen.append(item[0])
         else:
-            qs, m, _ = item
-            assert len(qs) <= 2 and all(q in tile for q in qs)
-            assert np.allclose(m.conj().T @ m, np.eye(m.shape[0]), atol=1e-12)
-            if len(qs) == 1:
-                orc.apply_one_qubit(got, n, m, qs[0])
-            else:
-                orc.apply_n_qubit(got, n, m, qs)
+            qs, mats, _, sq = item
+            assert all(q not in tile for q in sq) and len(sq) <= 3
+            if len(qs) + len(sq) > 2 or sq:
+                assert 1 <= len(qs) <= 2
+            for m in mats:
+                assert np.allclose(m.conj().T @ m, np.eye(m.shape[0]), atol=1e-12)
+            got = apply_selected(got, n, qs, mats, sq)
     assert len(set(seen)) == len(seen)
     assert len(plan) < len(ops)
     assert np.max(np.abs(got - want)) <= 1e-12
