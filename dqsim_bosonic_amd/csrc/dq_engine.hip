@@ -478,7 +478,7 @@ struct dq_state {
     int tile_bits = 12;
     int reg_tiles = 1;  // register-resident tile kernel for tiles of >= 9 qubits
     int last_grid = 0;  // grid of the last register-tile launch (partials it wrote)
-    int low_bits = 5;
+    int low_bits = 4;
     std::vector<HostOp> queue;
     // upload buffers
     void *d_upload = nullptr;
@@ -903,7 +903,7 @@ int dq_state_create(int n_qubits, int dtype, dq_state **out) {
     s->n = n_qubits;
     s->dtype = dtype;
     s->elem = dtype == DQ_C128 ? 16 : 8;
-    s->tile_bits = dtype == DQ_C128 ? 12 : 13;
+    s->tile_bits = 12;
     HIP_TRY(hipGetDevice(&s->device));
     auto cleanup = [&](int code, const std::string &msg) {
         dq_state_destroy(s);

@@ -22,7 +22,7 @@ struct PlanConfig {
     int n = 0;
     int fuse = 2;        // 0: one sweep per op; 1: program-order passes; 2: passes with lookahead
     int tile_bits = 12;  // qubits per tile
-    int low_bits = 5;    // qubits 0..low_bits-1 are in every tile (coalesced runs)
+    int low_bits = 4;    // qubits 0..low_bits-1 are in every tile (coalesced runs)
     int reg_tiles = 1;   // register-resident tiles for tiles of >= 9 qubits
     double fuse_min_cost = 128.0;  // fusion level 3: least cost of a fused block's gates
     int pair_fusion = 1;           // fusion levels 2-3: merge neighbouring gates on <= 2 qubits

@@ -93,20 +93,48 @@ __device__ __forceinline__ cplx<T> pick4(int e, cplx<T> m0, cplx<T> m1, cplx<T> 
     return cplx<T>{sel(b1, hi_re, lo_re), sel(b1, hi_im, lo_im)};
 }
 
+// Contracted-kernel arithmetic (FAST = 1): explicit FMA chains.  u * x is
+// (fma(ur, xr, -ui xi), fma(ur, xi, ui xr)); acc + u * x is four FMAs.  The
+// exact kernel (FAST = 0) keeps the reference's separately rounded products
+// and sums.
+template <typename T>
+__device__ __forceinline__ T dq_fma(T a, T b, T c) {
+    if constexpr (sizeof(T) == 8) return __builtin_fma(a, b, c);
+    else return __builtin_fmaf(a, b, c);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> fmul(cplx<T> u, cplx<T> x) {
+    return cplx<T>{dq_fma(u.re, x.re, -(u.im * x.im)), dq_fma(u.re, x.im, u.im * x.re)};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> fmac(cplx<T> acc, cplx<T> u, cplx<T> x) {
+    return cplx<T>{dq_fma(-u.im, x.im, dq_fma(u.re, x.re, acc.re)), dq_fma(u.im, x.re, dq_fma(u.re, x.im, acc.im))};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> frmac(cplx<T> acc, cplx<T> u, cplx<T> x) {  // u real
+    return cplx<T>{dq_fma(u.re, x.re, acc.re), dq_fma(u.re, x.im, acc.im)};
+}
+
 template <int P>
 __device__ __forceinline__ constexpr int pair_lo(int jj) {
     return ((jj >> P) << (P + 1)) | (jj & ((1 << P) - 1));
 }
 
 // 2x2 on register slot P, pairs (j0, j0 | 1 << P); out = u00*x + u01*y, u10*x + u11*y.
-template <typename T, int P, int V>
+template <typename T, int P, int V, int FAST>
 __device__ __forceinline__ void reg_dense1(cplx<T> (&a)[16], const cplx<T> *__restrict__ m) {
     const cplx<T> u00 = m[0], u01 = m[1], u10 = m[2], u11 = m[3];
 #pragma unroll
     for (int jj = 0; jj < 8; ++jj) {
         const int j0 = pair_lo<P>(jj), j1 = j0 | (1 << P);
         const cplx<T> x = a[j0], y = a[j1];
-        if (V == V_GEN) {
+        if (FAST && V == V_GEN) {
+            a[j0] = fmac(fmul(u01, y), u00, x);
+            a[j1] = fmac(fmul(u11, y), u10, x);
+        } else if (FAST && V == V_REAL) {
+            a[j0] = frmac(cplx<T>{u01.re * y.re, u01.re * y.im}, u00, x);
+            a[j1] = frmac(cplx<T>{u11.re * y.re, u11.re * y.im}, u10, x);
+        } else if (V == V_GEN) {
             a[j0] = cadd(cmul(u00, x), cmul(u01, y));
             a[j1] = cadd(cmul(u10, x), cmul(u11, y));
         } else if (V == V_REAL) {
@@ -123,7 +151,7 @@ __device__ __forceinline__ void reg_dense1(cplx<T> (&a)[16], const cplx<T> *__re
 }
 
 // Controlled 2x2 on slot P: the pair changes only where the control pattern holds.
-template <typename T, int P, int V>
+template <typename T, int P, int V, int FAST>
 __device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[16], const cplx<T> *__restrict__ m, const Layout<T> &L,
                                           unsigned cmask, unsigned cval) {
     const cplx<T> u00 = m[0], u01 = m[1], u10 = m[2], u11 = m[3];
@@ -135,6 +163,9 @@ __device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[16], const cplx<T> *__res
         if (V == V_ANTI) {
             n0 = cmul(u01, y);
             n1 = cmul(u10, x);
+        } else if (FAST) {
+            n0 = fmac(fmul(u01, y), u00, x);
+            n1 = fmac(fmul(u11, y), u10, x);
         } else {
             n0 = cadd(cmul(u00, x), cmul(u01, y));
             n1 = cadd(cmul(u10, x), cmul(u11, y));
@@ -146,7 +177,7 @@ __device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[16], const cplx<T> *__res
 }
 
 // 4x4 on register slots P (matrix MSB) and Q: acc = 0; acc += U[r][c] * v[c].
-template <typename T, int P, int Q, int V>
+template <typename T, int P, int Q, int V, int FAST>
 __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[16], const cplx<T> *__restrict__ m) {
     constexpr int LO = P < Q ? P : Q, HI = P < Q ? Q : P;
 #pragma unroll
@@ -162,6 +193,14 @@ __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[16], const cplx<T> *__re
             cplx<T> acc;
             if (V == V_DIAG) {
                 acc = cmul(m[r * 5], v[r]);
+            } else if (FAST && V == V_REAL) {
+                acc = cplx<T>{m[r * 4 + 3].re * v[3].re, m[r * 4 + 3].re * v[3].im};
+#pragma unroll
+                for (int c = 2; c >= 0; --c) acc = frmac(acc, m[r * 4 + c], v[c]);
+            } else if (FAST) {
+                acc = fmul(m[r * 4 + 3], v[3]);
+#pragma unroll
+                for (int c = 2; c >= 0; --c) acc = fmac(acc, m[r * 4 + c], v[c]);
             } else if (V == V_REAL) {
                 acc = rmul(m[r * 4], v[0]);
 #pragma unroll
@@ -176,68 +215,68 @@ __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[16], const cplx<T> *__re
     }
 }
 
-template <typename T, int P>
+template <typename T, int P, int FAST>
 __device__ __forceinline__ void dense1_v(cplx<T> (&a)[16], int v, const cplx<T> *m) {
     switch (v) {
-    case V_REAL: reg_dense1<T, P, V_REAL>(a, m); break;
-    case V_DIAG: reg_dense1<T, P, V_DIAG>(a, m); break;
-    case V_ANTI: reg_dense1<T, P, V_ANTI>(a, m); break;
-    default: reg_dense1<T, P, V_GEN>(a, m); break;
+    case V_REAL: reg_dense1<T, P, V_REAL, FAST>(a, m); break;
+    case V_DIAG: reg_dense1<T, P, V_DIAG, FAST>(a, m); break;
+    case V_ANTI: reg_dense1<T, P, V_ANTI, FAST>(a, m); break;
+    default: reg_dense1<T, P, V_GEN, FAST>(a, m); break;
     }
 }
 
-template <typename T>
+template <typename T, int FAST>
 __device__ __forceinline__ void dispatch_dense1(cplx<T> (&a)[16], int p, int v, const cplx<T> *m) {
     switch (p) {
-    case 0: dense1_v<T, 0>(a, v, m); break;
-    case 1: dense1_v<T, 1>(a, v, m); break;
-    case 2: dense1_v<T, 2>(a, v, m); break;
-    default: dense1_v<T, 3>(a, v, m); break;
+    case 0: dense1_v<T, 0, FAST>(a, v, m); break;
+    case 1: dense1_v<T, 1, FAST>(a, v, m); break;
+    case 2: dense1_v<T, 2, FAST>(a, v, m); break;
+    default: dense1_v<T, 3, FAST>(a, v, m); break;
     }
 }
 
-template <typename T, int P>
+template <typename T, int P, int FAST>
 __device__ __forceinline__ void ctrl1_v(cplx<T> (&a)[16], int v, const cplx<T> *m, const Layout<T> &L, unsigned cm,
                                         unsigned cv) {
-    if (v == V_ANTI) reg_ctrl1<T, P, V_ANTI>(a, m, L, cm, cv);
-    else reg_ctrl1<T, P, V_GEN>(a, m, L, cm, cv);
+    if (v == V_ANTI) reg_ctrl1<T, P, V_ANTI, FAST>(a, m, L, cm, cv);
+    else reg_ctrl1<T, P, V_GEN, FAST>(a, m, L, cm, cv);
 }
 
-template <typename T>
+template <typename T, int FAST>
 __device__ __forceinline__ void dispatch_ctrl1(cplx<T> (&a)[16], int p, int v, const cplx<T> *m, const Layout<T> &L,
                                                unsigned cmask, unsigned cval) {
     switch (p) {
-    case 0: ctrl1_v<T, 0>(a, v, m, L, cmask, cval); break;
-    case 1: ctrl1_v<T, 1>(a, v, m, L, cmask, cval); break;
-    case 2: ctrl1_v<T, 2>(a, v, m, L, cmask, cval); break;
-    default: ctrl1_v<T, 3>(a, v, m, L, cmask, cval); break;
+    case 0: ctrl1_v<T, 0, FAST>(a, v, m, L, cmask, cval); break;
+    case 1: ctrl1_v<T, 1, FAST>(a, v, m, L, cmask, cval); break;
+    case 2: ctrl1_v<T, 2, FAST>(a, v, m, L, cmask, cval); break;
+    default: ctrl1_v<T, 3, FAST>(a, v, m, L, cmask, cval); break;
     }
 }
 
-template <typename T, int P, int Q>
+template <typename T, int P, int Q, int FAST>
 __device__ __forceinline__ void dense2_v(cplx<T> (&a)[16], int v, const cplx<T> *m) {
     switch (v) {
-    case V_REAL: reg_dense2<T, P, Q, V_REAL>(a, m); break;
-    case V_DIAG: reg_dense2<T, P, Q, V_DIAG>(a, m); break;
-    default: reg_dense2<T, P, Q, V_GEN>(a, m); break;
+    case V_REAL: reg_dense2<T, P, Q, V_REAL, FAST>(a, m); break;
+    case V_DIAG: reg_dense2<T, P, Q, V_DIAG, FAST>(a, m); break;
+    default: reg_dense2<T, P, Q, V_GEN, FAST>(a, m); break;
     }
 }
 
-template <typename T>
+template <typename T, int FAST>
 __device__ __forceinline__ void dispatch_dense2(cplx<T> (&a)[16], int p, int q, int v, const cplx<T> *m) {
     switch (p * 4 + q) {
-    case 0 * 4 + 1: dense2_v<T, 0, 1>(a, v, m); break;
-    case 0 * 4 + 2: dense2_v<T, 0, 2>(a, v, m); break;
-    case 0 * 4 + 3: dense2_v<T, 0, 3>(a, v, m); break;
-    case 1 * 4 + 0: dense2_v<T, 1, 0>(a, v, m); break;
-    case 1 * 4 + 2: dense2_v<T, 1, 2>(a, v, m); break;
-    case 1 * 4 + 3: dense2_v<T, 1, 3>(a, v, m); break;
-    case 2 * 4 + 0: dense2_v<T, 2, 0>(a, v, m); break;
-    case 2 * 4 + 1: dense2_v<T, 2, 1>(a, v, m); break;
-    case 2 * 4 + 3: dense2_v<T, 2, 3>(a, v, m); break;
-    case 3 * 4 + 0: dense2_v<T, 3, 0>(a, v, m); break;
-    case 3 * 4 + 1: dense2_v<T, 3, 1>(a, v, m); break;
-    default: dense2_v<T, 3, 2>(a, v, m); break;
+    case 0 * 4 + 1: dense2_v<T, 0, 1, FAST>(a, v, m); break;
+    case 0 * 4 + 2: dense2_v<T, 0, 2, FAST>(a, v, m); break;
+    case 0 * 4 + 3: dense2_v<T, 0, 3, FAST>(a, v, m); break;
+    case 1 * 4 + 0: dense2_v<T, 1, 0, FAST>(a, v, m); break;
+    case 1 * 4 + 2: dense2_v<T, 1, 2, FAST>(a, v, m); break;
+    case 1 * 4 + 3: dense2_v<T, 1, 3, FAST>(a, v, m); break;
+    case 2 * 4 + 0: dense2_v<T, 2, 0, FAST>(a, v, m); break;
+    case 2 * 4 + 1: dense2_v<T, 2, 1, FAST>(a, v, m); break;
+    case 2 * 4 + 3: dense2_v<T, 2, 3, FAST>(a, v, m); break;
+    case 3 * 4 + 0: dense2_v<T, 3, 0, FAST>(a, v, m); break;
+    case 3 * 4 + 1: dense2_v<T, 3, 1, FAST>(a, v, m); break;
+    default: dense2_v<T, 3, 2, FAST>(a, v, m); break;
     }
 }
 
@@ -418,14 +457,14 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             }
             switch (op.kind) {
             case T_DENSE1:
-                dispatch_dense1<T>(a, op.p, op.variant, m);
+                dispatch_dense1<T, FAST>(a, op.p, op.variant, m);
                 break;
             case T_CTRL1:
                 if ((unsigned(tile) & op.ocmask) == op.ocval)
-                    dispatch_ctrl1<T>(a, op.p, op.variant, m, L, op.cmask, op.cval);
+                    dispatch_ctrl1<T, FAST>(a, op.p, op.variant, m, L, op.cmask, op.cval);
                 break;
             case T_DENSE2:
-                dispatch_dense2<T>(a, op.p, op.q, op.variant, m);
+                dispatch_dense2<T, FAST>(a, op.p, op.q, op.variant, m);
                 break;
             case T_DIAGT: {
                 // e = thread part | register part; the thread part picks the
