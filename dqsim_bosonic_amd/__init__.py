@@ -31,12 +31,14 @@ from ._core import (  # noqa: E402
     StatevectorSimulator,
     simulate_batch,
     simulate_monolithic,
+    simulate_monolithic_shots,
 )
 
 ENGINE_LIBRARY = _os.path.join(_HERE, "libdqsim_gpu.so")
 
 __all__ = [
     "simulate_batch",
+    "simulate_monolithic_shots",
     "StatevectorSimulator",
     "SimulationResult",
     "SimulationProfile",

@@ -27,6 +27,8 @@ EXPORTS = (
     "dq_get_fused_stats", "dq_set_fuse_min_cost", "dq_set_pair_fusion",
     "dq_batch_create", "dq_batch_destroy", "dq_batch_add_instance", "dq_batch_apply",
     "dq_batch_set_observable", "dq_batch_set_workers", "dq_batch_run", "dq_batch_stats",
+    "dq_shots_create", "dq_shots_destroy", "dq_shots_add_gate", "dq_shots_add_measure", "dq_shots_add_reset",
+    "dq_shots_add_conditional", "dq_shots_words", "dq_shots_run",
     "dq_last_error",
 )
 
@@ -71,6 +73,15 @@ def lib() -> ctypes.CDLL:
         L.dq_batch_set_workers.argtypes = [vp, ctypes.c_int]
         L.dq_batch_run.argtypes = [vp, _dp]
         L.dq_batch_stats.argtypes = [vp, u64p, u64p, _dp]
+        L.dq_shots_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.dq_shots_destroy.argtypes = [vp]
+        L.dq_shots_destroy.restype = None
+        L.dq_shots_add_gate.argtypes = [vp, _dp, _ip, ctypes.c_int]
+        L.dq_shots_add_measure.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.dq_shots_add_reset.argtypes = [vp, ctypes.c_int]
+        L.dq_shots_add_conditional.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
+        L.dq_shots_words.argtypes = [vp]
+        L.dq_shots_run.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, u64p, _dp]
         L.dq_state_copy.argtypes = [vp, vp]
         L.dq_compare_states.argtypes = [vp, vp, _dp, _dp, _dp, _dp]
         L.dq_apply_pauli_sum.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp]
@@ -256,3 +267,40 @@ class Batch:
         a, b, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
         _check(lib().dq_batch_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
         return a.value, b.value, t.value
+
+
+class Shots:
+    """Owning wrapper of a dq_shots program (batched shots)."""
+
+    def __init__(self, n: int, num_cbits: int):
+        h = ctypes.c_void_p()
+        _check(lib().dq_shots_create(n, num_cbits, ctypes.byref(h)))
+        self.h = h
+        self.num_cbits = num_cbits
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.dq_shots_destroy(self.h)
+            self.h = None
+
+    def gate(self, u: np.ndarray, qubits) -> None:
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        q = (ctypes.c_int * len(qubits))(*qubits)
+        _check(lib().dq_shots_add_gate(self.h, _d(u.view(np.float64)), q, len(qubits)))
+
+    def measure(self, qubit: int, cbit: int) -> None:
+        _check(lib().dq_shots_add_measure(self.h, qubit, cbit))
+
+    def reset(self, qubit: int) -> None:
+        _check(lib().dq_shots_add_reset(self.h, qubit))
+
+    def conditional(self, base: int, size: int, value: int, n_ops: int) -> None:
+        _check(lib().dq_shots_add_conditional(self.h, base, size, value, n_ops))
+
+    def run(self, base_seed: int, shots: int, timed: bool = False):
+        words = int(lib().dq_shots_words(self.h))
+        bits = np.zeros(max(1, shots * words), dtype=np.uint64)
+        secs = np.zeros(max(1, shots), dtype=np.float64) if timed else None
+        _check(lib().dq_shots_run(self.h, base_seed, shots, bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                  _d(secs) if timed else None))
+        return bits[: shots * words].reshape(shots, words), secs

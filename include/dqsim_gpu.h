@@ -187,6 +187,33 @@ int dq_batch_run(dq_batch *b, double *expect);
 int dq_batch_stats(const dq_batch *b, uint64_t *lds_instances, uint64_t *engine_instances,
                    double *lds_ms);
 
+/* ---- batched shots ----
+ * Replaces the per-shot loop of StatevectorSimulator::simulate_shots
+ * (src/monolithic/statevector/simulator.rs:215-322): a program of gates,
+ * measurements, resets and conditionals (the fused instruction list of
+ * src/types.rs:254-293, built by the caller) runs `shots` times from
+ * |0...0>, shot i drawing from ChaCha8Rng::seed_from_u64(base_seed + i)
+ * (wrapping).  bits_out receives, per shot, dq_shots_words() 64-bit words of
+ * classical bits (bit c of the shot's words = cbit c, 0 if never written);
+ * shot_seconds (optional) each shot's device time.  Gates apply as
+ * apply_one_qubit_seq / apply_n_qubit_seq (src/engine.rs:265-333) and
+ * measurements as measure_qubit_seq (:335-360), sequential p1 sum included,
+ * on the one-workgroup-per-shot path (n <= 13). */
+typedef struct dq_shots dq_shots;
+
+int dq_shots_create(int n_qubits, int num_cbits, dq_shots **out);
+void dq_shots_destroy(dq_shots *s);
+int dq_shots_add_gate(dq_shots *s, const double *u, const int *qubits, int k);
+int dq_shots_add_measure(dq_shots *s, int qubit, int cbit);
+int dq_shots_add_reset(dq_shots *s, int qubit);
+/* The next n_ops program ops run only if the classical bits
+ * creg_base..creg_base+creg_size-1 (bit b weighted 2^b) equal creg_value
+ * (simulator.rs:841-850). */
+int dq_shots_add_conditional(dq_shots *s, int creg_base, int creg_size, uint64_t creg_value, int n_ops);
+int dq_shots_words(const dq_shots *s);
+int dq_shots_run(dq_shots *s, uint64_t base_seed, uint64_t shots, uint64_t *bits_out,
+                 double *shot_seconds);
+
 const char *dq_last_error(void);
 
 #ifdef __cplusplus

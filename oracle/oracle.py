@@ -18,6 +18,10 @@ the reference around them, independently of the product's C++ decoder:
   simulator.rs:148-191 over ``marginal_probs`` / ``sample_counts``
   (src/engine.rs:368-433).
 * ``num_qubits`` restates src/types.rs:24-30.
+* ``simulate_shots_json`` restates ``StatevectorSimulator::simulate_shots``
+  (simulator.rs:215-322) with ``fuse_circuit`` / ``gate_matrix_1q`` /
+  ``matmul2x2`` / ``is_identity_2x2`` (src/types.rs:149-293) and
+  ``format_cbits`` (types.rs:43-51).
 
 Parity status of each piece is listed in the test modules that pin it.
 """
@@ -297,6 +301,22 @@ class OracleSimulator:
         else:
             self._nq(state, n, u, qubits)
 
+    # run_instruction_par (simulator.rs:638-866): the shots dispatcher; same
+    # arms with the sequential kernels, and `circuit-*` generic names are
+    # no-ops there (:825-827).
+    def _run_shot(self, state, n, inst, cbits, rng):
+        if inst["kind"] == "gate" and inst["name"].lower().startswith("circuit-"):
+            return
+        if inst["kind"] == "conditional":
+            c = inst["condition"]
+            actual = 0
+            for b in range(c["creg_size"]):
+                actual |= cbits.get(c["creg_base"] + b, 0) << b
+            if actual == c["creg_value"]:
+                self._run_shot(state, n, inst["op"], cbits, rng)
+            return
+        self._run(state, n, inst, cbits, rng, [])
+
     def _oq(self, state, n, u, q):
         if self.parallel and n >= 12:
             u = np.ascontiguousarray(u, dtype=np.complex128)
@@ -320,3 +340,164 @@ class OracleSimulator:
             lib().or_collapse(d, n, q, outcome, p1)
         p1s.append(p1)
         return outcome
+
+
+# -- shots: simulate_shots (simulator.rs:215-322), fuse_circuit (types.rs:149-293)
+
+def _cm(a, b):
+    """num-complex multiply on (re, im) tuples: separately rounded products."""
+    return (a[0] * b[0] - a[1] * b[1], a[0] * b[1] + a[1] * b[0])
+
+
+def _ca(a, b):
+    return (a[0] + b[0], a[1] + b[1])
+
+
+def fused_gate_matrix_1q(inst: dict):
+    """gate_matrix_1q (types.rs:164-230): the fusable single-qubit kinds and
+    the matrices the shots path multiplies; None for anything else.  Returns
+    ((re, im) x 4) row-major."""
+    import math
+    k = inst["kind"]
+    r = lambda x: (x, 0.0)  # noqa: E731
+    s2 = 0.70710678118654752440
+    neg = lambda a: (-a[0], -a[1])  # noqa: E731
+    if k in ("id", "u0"):
+        return (r(1.0), r(0.0), r(0.0), r(1.0))
+    if k == "x":
+        return (r(0.0), r(1.0), r(1.0), r(0.0))
+    if k == "y":
+        return (r(0.0), (0.0, -1.0), (0.0, 1.0), r(0.0))
+    if k == "z":
+        return (r(1.0), r(0.0), r(0.0), r(-1.0))
+    if k == "h":
+        return (r(s2), r(s2), r(s2), r(-s2))
+    if k == "s":
+        return (r(1.0), r(0.0), r(0.0), (0.0, 1.0))
+    if k == "sdg":
+        return (r(1.0), r(0.0), r(0.0), (0.0, -1.0))
+    if k == "t":
+        return (r(1.0), r(0.0), r(0.0), (s2, s2))
+    if k == "tdg":
+        return (r(1.0), r(0.0), r(0.0), (s2, -s2))
+    if k == "sx":
+        return ((0.5, 0.5), (0.5, -0.5), (0.5, -0.5), (0.5, 0.5))
+    if k == "sxdg":
+        return ((0.5, -0.5), (0.5, 0.5), (0.5, 0.5), (0.5, -0.5))
+    if k in ("u3", "u", "u2"):
+        if k == "u2":
+            theta, phi, lam = math.pi / 2.0, float(inst["phi"]), float(inst["lam"])
+        else:
+            theta, phi, lam = float(inst["theta"]), float(inst["phi"]), float(inst["lam"])
+        ct, st = math.cos(theta / 2.0), math.sin(theta / 2.0)
+        return (r(ct), neg(_cm((math.cos(lam), math.sin(lam)), r(st))),
+                _cm((math.cos(phi), math.sin(phi)), r(st)),
+                _cm((math.cos(phi + lam), math.sin(phi + lam)), r(ct)))
+    if k in ("u1", "p"):
+        lam = float(inst["lam"])
+        return (r(1.0), r(0.0), r(0.0), (math.cos(lam), math.sin(lam)))
+    if k == "rx":
+        th = float(inst["theta"])
+        cv, sv = math.cos(th / 2.0), math.sin(th / 2.0)
+        return (r(cv), (0.0, -sv), (0.0, -sv), r(cv))
+    if k == "ry":
+        th = float(inst["theta"])
+        cv, sv = math.cos(th / 2.0), math.sin(th / 2.0)
+        return (r(cv), r(-sv), r(sv), r(cv))
+    if k == "rz":
+        ph = float(inst["phi"])
+        return ((math.cos(-ph / 2.0), math.sin(-ph / 2.0)), r(0.0), r(0.0),
+                (math.cos(ph / 2.0), math.sin(ph / 2.0)))
+    return None
+
+
+def matmul2x2(a, b):
+    """types.rs:149-160, a * b."""
+    return (_ca(_cm(a[0], b[0]), _cm(a[1], b[2])), _ca(_cm(a[0], b[1]), _cm(a[1], b[3])),
+            _ca(_cm(a[2], b[0]), _cm(a[3], b[2])), _ca(_cm(a[2], b[1]), _cm(a[3], b[3])))
+
+
+def is_identity_2x2(m) -> bool:
+    """types.rs:239-244 (num-complex norm = hypot)."""
+    import math
+    return (math.hypot(m[0][0] - 1.0, m[0][1]) < 1e-10 and math.hypot(*m[1]) < 1e-10
+            and math.hypot(*m[2]) < 1e-10 and math.hypot(m[3][0] - 1.0, m[3][1]) < 1e-10)
+
+
+def inst_qubits(inst: dict) -> list:
+    """Instruction::qubits (types.rs:348-418)."""
+    k = inst["kind"]
+    if k == "conditional":
+        return inst_qubits(inst["op"])
+    if k in ("barrier", "classical"):
+        return []
+    if k == "gate":
+        return list(inst["qubits"])
+    if k in ("measure", "reset"):
+        return [inst["qubit"]]
+    return [int(inst[f]) for f in _QUBIT_FIELDS[k]]
+
+
+def fuse_circuit(insts: list) -> list:
+    """types.rs:254-293: ("fused", qubit, matrix) or ("orig", inst).  The
+    qubits still pending at the end are flushed in ascending order (the
+    reference iterates a HashMap there; those gates act on distinct qubits
+    after the last measurement, so the order changes no outcome)."""
+    pending: dict = {}
+    out = []
+    ident = ((1.0, 0.0), (0.0, 0.0), (0.0, 0.0), (1.0, 0.0))
+
+    def flush(q):
+        m = pending.pop(q, None)
+        if m is not None and not is_identity_2x2(m):
+            out.append(("fused", q, m))
+
+    for inst in insts:
+        m1 = fused_gate_matrix_1q(inst) if inst["kind"] in _QUBIT_FIELDS and \
+            len(_QUBIT_FIELDS[inst["kind"]]) == 1 else None
+        if m1 is not None:
+            q = int(inst["qubit"])
+            pending[q] = matmul2x2(m1, pending.get(q, ident))
+            continue
+        for q in inst_qubits(inst):
+            flush(q)
+        out.append(("orig", inst))
+    for q in sorted(pending):
+        flush(q)
+    return out
+
+
+def format_cbits(cbits: dict, num_cbits: int) -> str:
+    """types.rs:43-51."""
+    return "".join(str(cbits.get(i, 0)) for i in range(num_cbits - 1, -1, -1))
+
+
+def num_cbits(circ: dict) -> int:
+    """types.rs:32-39."""
+    return max((r["base"] + r["size"] for r in circ.get("cregs", {}).values()), default=0)
+
+
+def simulate_shots_json(text: str, shots: int, base_seed: int) -> dict:
+    """StatevectorSimulator::simulate_shots (simulator.rs:215-322): each shot
+    from |0..0> with ChaCha8Rng::seed_from_u64(base_seed + i), the fused list
+    through run_fused_par (:870-884) with the sequential kernels."""
+    circ = json.loads(text)
+    n = num_qubits(circ)
+    ncb = num_cbits(circ)
+    fused = fuse_circuit(circ["instructions"])
+    sim = OracleSimulator(seed=0, parallel=False)
+    counts: dict[str, int] = {}
+    for i in range(shots):
+        state = np.zeros(1 << n, dtype=np.complex128)
+        state[0] = 1.0
+        cbits: dict[int, int] = {}
+        rng = Rng((base_seed + i) & (2**64 - 1))
+        for item in fused:
+            if item[0] == "fused":
+                u = np.array([complex(*e) for e in item[2]], dtype=np.complex128).reshape(2, 2)
+                apply_one_qubit(state, n, u, item[1])
+            else:
+                sim._run_shot(state, n, item[1], cbits, rng)
+        key = format_cbits(cbits, ncb)
+        counts[key] = counts.get(key, 0) + 1
+    return counts

@@ -4,6 +4,9 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
+TESTS = os.path.dirname(os.path.abspath(__file__))
+if TESTS not in sys.path:  # shared circuit builders of the test modules
+    sys.path.insert(1, TESTS)
 
 
 def pytest_configure(config):
