@@ -40,6 +40,11 @@ batch (batching form): 4096 hardware-efficient-ansatz circuits, n ~ U{10..24},
 depth ~ U{8..64} (ry, rz on every qubit then a cx ring per layer), <sum_i
 Z_i Z_{i+1}> of each, complex128 and complex64: circuits/s, and the unfused
 algorithmic bytes (every gate reading and writing the whole state) per second.
+
+shots (batched shots): simulate_shots of a 12-qubit random-block circuit with
+mid-circuit measure / reset / conditional and a final measurement of every
+qubit; shots/s on the GPU, and the oracle's sequential restatement on a
+bounded sample as its CPU baseline.
 """
 from __future__ import annotations
 
@@ -168,6 +173,37 @@ def batch_leg(count: int, seed: int, workers: int) -> dict:
     return out
 
 
+def shots_leg(shots: int, seed: int, cpu_seconds: float) -> dict:
+    """simulate_shots of a 12-qubit random-block circuit with mid-circuit
+    measure / reset / conditional and a final measurement of every qubit."""
+    from dqsim_bosonic_amd import StatevectorSimulator
+    from dqsim_bosonic_amd import workloads as W
+
+    circ = W.shots_blocks(12, 20, seed)
+    sim = StatevectorSimulator(seed=seed)
+    sim.simulate_shots(circ, shots=256)  # warm-up
+    t0 = time.perf_counter()
+    counts = sim.simulate_shots(circ, shots=shots)
+    el = time.perf_counter() - t0
+    out = {"workload": f"shots_blocks 12q depth 20 ({W.gate_count(circ)} gates, "
+                       f"{sum(1 for i in circ.instructions if i.kind == 'measure')} measurements per shot)",
+           "shots": shots, "shots_per_s": round(shots / el, 1), "seconds": round(el, 3),
+           "distinct_outcomes": len(counts)}
+    if cpu_seconds > 0:
+        from oracle import oracle as orc
+
+        text = circ.model_dump_json()
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            orc.simulate_shots_json(text, 8, seed + done)
+            done += 8
+        cpu_el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"shots_per_s": round(done / cpu_el, 2), "cores": 1, "kind": "port",
+                               "sample": f"{done} shots through the oracle's sequential restatement "
+                                         f"({cpu_el:.1f} s)"}
+    return out
+
+
 def precision_leg(n: int, depth: int, seed: int, steps: int) -> dict:
     from dqsim_bosonic_amd import _core
     from dqsim_bosonic_amd import workloads as W
@@ -236,6 +272,8 @@ def main() -> int:
     ap.add_argument("--ps-subset", type=int, default=8)
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-shots", action="store_true")
+    ap.add_argument("--shots", type=int, default=100000)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--precision-qubits", type=int, default=32)
@@ -309,6 +347,12 @@ def main() -> int:
         batch = batch_leg(args.batch_count, args.seed, args.batch_workers)
         log(f"[bench] batch {batch}")
 
+    shots = None
+    if not args.no_shots:
+        log("[bench] shots ...")
+        shots = shots_leg(args.shots, args.seed, 0.0 if args.no_cpu_baseline else 5.0)
+        log(f"[bench] shots {shots}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -328,6 +372,7 @@ def main() -> int:
         "adjoint": adj,
         "precision": prec,
         "batch": batch,
+        "shots": shots,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -293,7 +293,9 @@ inline std::vector<const HostOp *> fuse_blocks(const std::vector<const HostOp *>
                                                int b, double min_cost, std::deque<HostOp> &store) {
     std::vector<const HostOp *> out, rest(in), next;
     auto tile_op = [&](const HostOp &h) {
-        if (h.kind == K_COLLAPSE || h.k > 5) return false;
+        // ops holding one matrix per outer-qubit value (pair fusion) act on
+        // qubits outside the tile: they stay out of blocks
+        if (h.kind == K_COLLAPSE || h.k > 5 || h.nsel > 0) return false;
         for (int j = 0; j < h.k; ++j)
             if (pos[h.q[j]] >= kOuterBit) return false;
         return true;

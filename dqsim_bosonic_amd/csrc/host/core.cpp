@@ -1311,8 +1311,10 @@ PYBIND11_MODULE(_core, m) {
         std::vector<const dq::HostOp *> in;
         for (const dq::HostOp &h : queue) in.push_back(&h);
         std::deque<dq::HostOp> store, pstore;
-        auto eff = max_qubits == 2 ? dq::fuse_pairs(in, pos, pstore)
-                                   : dq::fuse_blocks(in, pos, tq, int(tile_qubits.size()), min_cost, store);
+        // max_qubits 2: pair fusion; 5: blocks; 7: pairs, then blocks (as the planner at fusion level 3)
+        auto eff = max_qubits == 5 ? dq::fuse_blocks(in, pos, tq, int(tile_qubits.size()), min_cost, store)
+                                   : dq::fuse_pairs(in, pos, pstore);
+        if (max_qubits == 7) eff = dq::fuse_blocks(eff, pos, tq, int(tile_qubits.size()), min_cost, store);
         py::list out;
         for (const dq::HostOp *h : eff) {
             if (h >= queue.data() && h < queue.data() + queue.size()) {

@@ -12,6 +12,8 @@ Each generator returns a ``circuit.Circuit`` in the reference's wire schema
   ``gate`` kind) between the two halves of the register; after every 10th
   layer, measure two random qubits, reset them, and apply one x conditioned
   on the first measured bit.
+* ``shots_blocks``: ``random_blocks`` plus a final measurement of every qubit,
+  the shots workload.
 * ``hea`` / ``hea_batch``: hardware-efficient ansatz circuits for batching:
   per layer ry then rz on every qubit, then a cx ring cx(i, i+1), i = 0..n-2,
   closed by cx(n-1, 0); angles U[0, 2pi).  ``hea_batch`` draws 4096 widths
@@ -163,3 +165,15 @@ def hea_batch(count: int = 4096, seed: int = 42, n_range=(10, 24), d_range=(8, 6
     """The same circuits as ``hea_batch_arrays`` as circuit objects."""
     ns, ds, angles = hea_batch_arrays(count, seed, n_range, d_range)
     return [hea(int(n), int(d), a) for n, d, a in zip(ns, ds, angles)]
+
+
+def shots_blocks(n: int = 12, depth: int = 20, seed: int = 42) -> C.Circuit:
+    """``random_blocks`` (mid-circuit measure / reset / conditional x every 10
+    layers) followed by a measurement of every qubit into its own cbit; the
+    classical register holds those n bits and the mid-circuit ones."""
+    base = random_blocks(n, depth, seed=seed)
+    mid = sum(1 for i in base.instructions if i.kind == "measure")
+    ins = list(base.instructions)
+    for q in range(n):
+        ins.append(C.MeasureInstruction(qubit=q, cbit=mid + q))
+    return C.circuit(n, ins, mid + n)

@@ -197,3 +197,27 @@ def test_pair_fusion_on_layered_circuit():
             ops.append(([q, q + 1], orc.gate_matrix("cz", []).ravel(), []))
     plan = core._fuse_blocks(list(range(n)), ops, max_qubits=2)
     assert len(plan) <= len(ops) // 2, (len(plan), len(ops))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_pairs_then_blocks_equal_program(seed):
+    """Fusion level 3's pipeline: pair fusion, then 5-qubit blocks, on ops
+    that touch outer qubits diagonally (selector ops must stay unblocked)."""
+    n, tile = 10, [0, 1, 2, 3, 4, 5, 6, 7]
+    rng = np.random.default_rng(300 + seed)
+    ops = pass_ops(n, tile, 90, rng)
+    plan = core._fuse_blocks(tile, [(qs, u.ravel(), cv) for qs, u, cv in ops], min_cost=0.0, max_qubits=7)
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    want = psi.copy()
+    for qs, u, cv in ops:
+        want = apply_op(want, n, qs, u, cv)
+    got = psi.copy()
+    for item in plan:
+        if len(item) == 1:
+            qs, u, cv = ops[item[0]]
+            got = apply_op(got, n, qs, u, cv)
+        else:
+            qs, mats, _, sq = item
+            got = apply_selected(got, n, qs, mats, sq)
+    assert np.max(np.abs(got - want)) <= 1e-12

@@ -236,3 +236,24 @@ def test_pair_fusion_then_blocks_match_oracle(fuse):
     want = oracle_run(n, prog, psi)
     st = run_c64(n, prog, psi, fuse=fuse, tile_bits=12, min_cost=0.0, pair=True)
     assert np.max(np.abs(st.get() - want)) <= AMP_TOL
+
+
+def test_blocks_with_outer_qubit_gates():
+    """Gates across tile and outer qubits: pair fusion gives ops with one
+    matrix per outer-qubit value; those must stay out of 5-qubit blocks."""
+    n = 16
+    rng = np.random.default_rng(41)
+    psi = random_state(n, rng)
+    prog = []
+    for layer in range(12):
+        for q in range(n):
+            g = GATES_1Q[int(rng.integers(len(GATES_1Q)))]
+            prog.append(("1q", orc.gate_matrix(g, rng.uniform(-4, 4, NPARAM.get(g, 0))), [q]))
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            g = ["cz", "cp", "rzz"][int(rng.integers(3))]
+            prog.append(("nq", orc.gate_matrix(g, rng.uniform(-4, 4, NPARAM.get(g, 0))), [int(perm[i]), int(perm[i + 1])]))
+    want = oracle_run(n, prog, psi)
+    for cost in (0.0, 128.0):
+        st = run_c64(n, prog, psi, fuse=3, tile_bits=12, min_cost=cost, pair=True)
+        assert np.max(np.abs(st.get() - want)) <= AMP_TOL
