@@ -1365,6 +1365,194 @@ int dq_pauli_inner(dq_state *bra, dq_state *ket, int n_terms, const uint64_t *xm
     return DQ_OK;
 }
 
+// ---- top-k of the distribution (lazy results; no reference counterpart) ----
+//
+// The k most probable basis states without copying the distribution to the
+// host: a radix select over the bit patterns of p = re^2 + im^2 (as for
+// probabilities(); non-negative doubles order like their bit patterns).
+// Each level histograms 11 bits of the keys that share the prefix chosen so
+// far (a read of the state), until the keys at or above the threshold are
+// few enough to collect; ties at the k-th value go to the smaller indices.
+
+extern "C++" {
+namespace {
+
+constexpr int kTopBins = 2048;
+
+__global__ void k_zero_bytes(unsigned char *__restrict__ p, size_t n) {
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+template <typename T>
+__device__ __forceinline__ uint64_t prob_key(const cplx<T> &a, double &p) {
+    const double re = double(a.re), im = double(a.im);
+    p = re * re + im * im;
+    return uint64_t(__double_as_longlong(p));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_topk_hist(const cplx<T> *__restrict__ amps, uint64_t count, int shift,
+                                                        int pshift, uint64_t prefix, unsigned mask,
+                                                        unsigned *__restrict__ hist) {
+    __shared__ unsigned h[kTopBins];
+    for (int i = threadIdx.x; i < kTopBins; i += kThreads) h[i] = 0;
+    __syncthreads();
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        double p;
+        const uint64_t key = prob_key(amps[i], p);
+        if (pshift >= 64 || (key >> pshift) == prefix) atomicAdd(&h[unsigned(key >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTopBins; i += kThreads)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// Collects (index, p) of the keys >= lo (and == lo only below index `below`
+// when tie_only is set).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_topk_collect(const cplx<T> *__restrict__ amps, uint64_t count,
+                                                           uint64_t lo, int tie_only, uint64_t below,
+                                                           uint64_t *__restrict__ idx, double *__restrict__ val,
+                                                           unsigned long long *__restrict__ counter, uint64_t cap) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        double p;
+        const uint64_t key = prob_key(amps[i], p);
+        const bool take = tie_only ? (key == lo && i < below) : (key >= lo);
+        if (take) {
+            const unsigned long long pos = atomicAdd(counter, 1ull);
+            if (pos < cap) {
+                idx[pos] = i;
+                val[pos] = p;
+            }
+        }
+    }
+}
+
+// Counts the keys equal to `key_eq` at indices below `below`.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_topk_count_ties(const cplx<T> *__restrict__ amps, uint64_t count,
+                                                              uint64_t key_eq, uint64_t below,
+                                                              unsigned long long *__restrict__ counter) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    unsigned long long c = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count && i < below; i += stride) {
+        double p;
+        c += prob_key(amps[i], p) == key_eq;
+    }
+    if (c) atomicAdd(counter, c);
+}
+
+template <typename T>
+int top_k_impl(dq_state *s, uint64_t k, uint64_t *idx_out, double *p_out) {
+    const uint64_t count = amp_count(s);
+    const cplx<T> *amps = static_cast<const cplx<T> *>(s->amps);
+    const uint64_t cap = std::max<uint64_t>(uint64_t(1) << 22, 2 * k);
+    const size_t hist_b = kTopBins * sizeof(unsigned), cnt_b = 256;
+    const size_t idx_b = cap * sizeof(uint64_t), val_b = cap * sizeof(double);
+    unsigned char *d = nullptr;
+    if (hipMalloc(&d, hist_b + cnt_b + idx_b + val_b) != hipSuccess) return fail(DQ_ERR_OOM, "hipMalloc for top-k failed");
+    struct Free {
+        unsigned char *p;
+        ~Free() { (void)hipFree(p); }
+    } guard{d};
+    unsigned *d_hist = reinterpret_cast<unsigned *>(d);
+    unsigned long long *d_cnt = reinterpret_cast<unsigned long long *>(d + hist_b);
+    uint64_t *d_idx = reinterpret_cast<uint64_t *>(d + hist_b + cnt_b);
+    double *d_val = reinterpret_cast<double *>(d + hist_b + cnt_b + idx_b);
+    const int blocks = std::min<int>(4096, grid_for(count));
+    std::vector<unsigned> hist(kTopBins);
+    auto zero = [&](void *p, size_t b) -> int {
+        hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(kThreads), 0, s->stream, static_cast<unsigned char *>(p), b);
+        HIP_TRY(hipGetLastError());
+        return DQ_OK;
+    };
+    auto collect = [&](uint64_t lo, int tie_only, uint64_t below, std::vector<std::pair<uint64_t, double>> &out) -> int {
+        if (int rc = zero(d_cnt, sizeof(unsigned long long))) return rc;
+        hipLaunchKernelGGL(k_topk_collect<T>, dim3(blocks), dim3(kThreads), 0, s->stream, amps, count, lo, tie_only,
+                           below, d_idx, d_val, d_cnt, cap);
+        HIP_TRY(hipGetLastError());
+        unsigned long long got = 0;
+        HIP_TRY(hipMemcpyAsync(&got, d_cnt, sizeof got, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        const uint64_t m = std::min<uint64_t>(got, cap);
+        std::vector<uint64_t> ii(m);
+        std::vector<double> vv(m);
+        HIP_TRY(hipMemcpyAsync(ii.data(), d_idx, m * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipMemcpyAsync(vv.data(), d_val, m * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (uint64_t j = 0; j < m; ++j) out.emplace_back(ii[j], vv[j]);
+        return DQ_OK;
+    };
+    uint64_t above = 0, prefix = 0;  // keys known above the bracket; key bits fixed so far
+    int pshift = 64;
+    std::vector<std::pair<uint64_t, double>> cand;
+    static const int shifts[6] = {53, 42, 31, 20, 9, 0};
+    for (int level = 0; level < 6; ++level) {
+        const int shift = shifts[level], width = pshift - shift;
+        if (int rc = zero(d_hist, hist_b)) return rc;
+        hipLaunchKernelGGL(k_topk_hist<T>, dim3(blocks), dim3(kThreads), 0, s->stream, amps, count, shift, pshift,
+                           prefix, (1u << width) - 1u, d_hist);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(hist.data(), d_hist, hist_b, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        int b = (1 << width) - 1;
+        uint64_t acc = above;
+        for (; b > 0; --b) {
+            if (acc + hist[size_t(b)] >= k) break;
+            acc += hist[size_t(b)];
+        }
+        above = acc;
+        prefix = (prefix << width) | uint64_t(b);
+        pshift = shift;
+        const uint64_t lo = prefix << shift;
+        if (above + hist[size_t(b)] <= cap) {  // few enough: collect every key >= lo
+            if (int rc = collect(lo, 0, 0, cand)) return rc;
+            break;
+        }
+        if (shift == 0) {  // key fully resolved: all keys above it, then the smallest-index ties
+            if (int rc = collect(lo + 1, 0, 0, cand)) return rc;
+            const uint64_t need = k - cand.size();
+            uint64_t lo_i = 0, hi_i = count;  // smallest bound with `need` ties below it
+            while (lo_i < hi_i) {
+                const uint64_t mid = lo_i + (hi_i - lo_i) / 2;
+                if (int rc = zero(d_cnt, sizeof(unsigned long long))) return rc;
+                hipLaunchKernelGGL(k_topk_count_ties<T>, dim3(blocks), dim3(kThreads), 0, s->stream, amps, count, lo,
+                                   mid, d_cnt);
+                HIP_TRY(hipGetLastError());
+                unsigned long long c = 0;
+                HIP_TRY(hipMemcpyAsync(&c, d_cnt, sizeof c, hipMemcpyDeviceToHost, s->stream));
+                HIP_TRY(hipStreamSynchronize(s->stream));
+                if (c >= need) hi_i = mid;
+                else lo_i = mid + 1;
+            }
+            if (int rc = collect(lo, 1, lo_i, cand)) return rc;
+            break;
+        }
+    }
+    std::sort(cand.begin(), cand.end(), [](const std::pair<uint64_t, double> &x, const std::pair<uint64_t, double> &y) {
+        return x.second != y.second ? x.second > y.second : x.first < y.first;
+    });
+    if (cand.size() < k) return fail(DQ_ERR_DEVICE, "top-k selection collected too few candidates");
+    for (uint64_t j = 0; j < k; ++j) {
+        idx_out[j] = cand[j].first;
+        p_out[j] = cand[j].second;
+    }
+    return DQ_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int dq_top_k(dq_state *s, uint64_t k, uint64_t *idx_out, double *p_out) {
+    if (!s || !idx_out || !p_out) return fail(DQ_ERR_ARG, "null argument");
+    if (k < 1 || k > (uint64_t(1) << 20) || k > amp_count(s))
+        return fail(DQ_ERR_ARG, "k must be in [1, min(2^20, 2^n)]");
+    if (int rc = flush(s)) return rc;
+    return s->dtype == DQ_C128 ? top_k_impl<double>(s, k, idx_out, p_out) : top_k_impl<float>(s, k, idx_out, p_out);
+}
+
 int dq_compare_states(dq_state *a, dq_state *b, double *max_abs, double *inner_re, double *inner_im,
                       double *tvd) {
     if (!a || !b || !max_abs || !inner_re || !inner_im || !tvd) return fail(DQ_ERR_ARG, "null argument");

@@ -39,6 +39,7 @@
 #include "../../../include/dqsim_gpu.h"
 #include "circuit.hpp"
 #include "gates.hpp"
+#include "opstream.hpp"
 #include "rng.hpp"
 #include "../dq_plan.hpp"
 
@@ -159,6 +160,36 @@ class SimulationResult {
         std::vector<double> probs(size_t(1) << qs.size());
         check(dq_marginal_probs(st_->s, qs.data(), int(qs.size()), probs.data()), "marginal probabilities");
         return probs;
+    }
+
+    // Lazy results (no reference counterpart): the distribution as an array
+    // (index j as in probabilities(), no dict) ...
+    py::array_t<double> probabilities_array(const std::optional<std::vector<int64_t>> &qubits) const {
+        const std::vector<int> qs = qubits_list(qubits, n_);
+        if (qs.size() > 34) throw py::value_error("at most 34 qubits");
+        py::array_t<double> out(static_cast<py::ssize_t>(size_t(1) << qs.size()));
+        double *p = out.mutable_data();
+        {
+            py::gil_scoped_release nogil;
+            check(dq_marginal_probs(st_->s, qs.data(), int(qs.size()), p), "marginal probabilities");
+        }
+        return out;
+    }
+
+    // ... and the k most probable basis states over all qubits, selected on
+    // the device: [(index, probability)], decreasing, ties by index.
+    py::list top_k(uint64_t k) const {
+        std::vector<uint64_t> idx(static_cast<size_t>(k));
+        std::vector<double> p(static_cast<size_t>(k));
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = dq_top_k(st_->s, k, idx.data(), p.data());
+        }
+        check(rc, "top_k");
+        py::list out;
+        for (size_t j = 0; j < idx.size(); ++j) out.append(py::make_tuple(py::int_(idx[j]), py::float_(p[j])));
+        return out;
     }
 
     // simulator.rs:148-159
@@ -587,6 +618,35 @@ class StatevectorSimulator {
         } catch (const JsonError &e) {
             throw py::value_error(std::string("Circuit JSON parse error: ") + e.what());
         }
+        return run_circuit(c);
+    }
+
+    // The same simulation from the opcode-stream wire format
+    // (include/dqsim_opstream.h): no JSON on either side.
+    SimulationResult simulate_opstream(py::array records, uint64_t num_qubits, uint64_t num_cbits,
+                                       const std::vector<std::string> &names) const {
+        return run_circuit(opstream_circuit(records, num_qubits, num_cbits, names));
+    }
+
+    py::dict simulate_shots_opstream(py::array records, uint64_t num_qubits, uint64_t num_cbits,
+                                     const std::vector<std::string> &names, uint64_t shots, bool profile) const {
+        return shots_circuit(opstream_circuit(records, num_qubits, num_cbits, names), shots, profile,
+                             Clock::now());
+    }
+
+    static Circuit opstream_circuit(py::array records, uint64_t num_qubits, uint64_t num_cbits,
+                                    const std::vector<std::string> &names) {
+        if (records.itemsize() != py::ssize_t(sizeof(dq_op_record)) || !(records.flags() & py::array::c_style))
+            throw py::value_error("records must be a C-contiguous array of 64-byte dq_op_record");
+        try {
+            return decode_opstream(static_cast<const dq_op_record *>(records.data()), size_t(records.size()),
+                                   num_qubits, num_cbits, names);
+        } catch (const OpstreamError &e) {
+            throw py::value_error(std::string("opcode stream error: ") + e.what());
+        }
+    }
+
+    SimulationResult run_circuit(const Circuit &c) const {
         const uint64_t n64 = c.num_qubits();
         if (n64 > 40) throw py::value_error("circuit has " + std::to_string(n64) + " qubits; at most 40 are supported");
         const int n = int(n64);
@@ -619,6 +679,10 @@ class StatevectorSimulator {
         } catch (const JsonError &e) {
             throw py::value_error(std::string("Circuit JSON parse error: ") + e.what());
         }
+        return shots_circuit(c, shots, profile, total_t0);
+    }
+
+    py::dict shots_circuit(const Circuit &c, uint64_t shots, bool profile, Clock::time_point total_t0) const {
         const uint64_t n = c.num_qubits();
         if (n < 1 || n > 34) throw py::value_error("circuit has " + std::to_string(n) + " qubits; 1..34 are supported");
         const int num_cbits = int(c.num_cbits());
@@ -1151,6 +1215,8 @@ PYBIND11_MODULE(_core, m) {
         .def_property_readonly("classical_bits", &SimulationResult::classical_bits)
         .def_property_readonly("profile", &SimulationResult::profile)
         .def("probabilities", &SimulationResult::probabilities, py::arg("qubits") = py::none())
+        .def("probabilities_array", &SimulationResult::probabilities_array, py::arg("qubits") = py::none())
+        .def("top_k", &SimulationResult::top_k, py::arg("k"))
         .def("counts", &SimulationResult::counts, py::arg("shots") = 1000, py::arg("qubits") = py::none(),
              py::arg("seed") = py::none())
         .def("fidelity", &SimulationResult::fidelity, py::arg("other"));
@@ -1161,7 +1227,12 @@ PYBIND11_MODULE(_core, m) {
              py::arg("low_bits") = 0)
         .def("simulate", &StatevectorSimulator::simulate, py::arg("circuit"))
         .def("simulate_shots", &StatevectorSimulator::simulate_shots, py::arg("circuit"), py::arg("shots") = 1000,
-             py::arg("profile") = false);
+             py::arg("profile") = false)
+        .def("simulate_opstream", &StatevectorSimulator::simulate_opstream, py::arg("records"), py::arg("num_qubits"),
+             py::arg("num_cbits") = 0, py::arg("names") = std::vector<std::string>{})
+        .def("simulate_shots_opstream", &StatevectorSimulator::simulate_shots_opstream, py::arg("records"),
+             py::arg("num_qubits"), py::arg("num_cbits") = 0, py::arg("names") = std::vector<std::string>{},
+             py::arg("shots") = 1000, py::arg("profile") = false);
 
     // simulator.rs (crate root) simulate_monolithic_shots
     m.def(
@@ -1245,6 +1316,25 @@ PYBIND11_MODULE(_core, m) {
         std::vector<uint64_t> v(count);
         for (auto &x : v) x = r.next_u64();
         return v;
+    });
+    auto listing = [](const Circuit &c) {
+        py::list out;
+        std::function<py::dict(const Instr &)> conv = [&](const Instr &in) {
+            py::dict d;
+            d["kind"] = in.kind;
+            d["qubits"] = in.qubits;
+            d["params"] = in.params;
+            d["name"] = in.name;
+            d["cbit"] = in.cbit;
+            d["condition"] = py::make_tuple(in.creg_base, in.creg_size, in.creg_value);
+            if (in.op) d["op"] = conv(*in.op);
+            return d;
+        };
+        for (const Instr &in : c.instructions) out.append(conv(in));
+        return py::make_tuple(c.num_qubits(), c.num_cbits(), out);
+    };
+    m.def("_decode_opstream", [listing](py::array records, uint64_t nq, uint64_t ncb, std::vector<std::string> names) {
+        return listing(StatevectorSimulator::opstream_circuit(records, nq, ncb, names));
     });
     m.def("_decode", [](const std::string &text) {
         Circuit c;

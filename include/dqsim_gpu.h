@@ -107,6 +107,12 @@ int dq_apply_pauli_sum(dq_state *dst, dq_state *src, int n_terms, const uint64_t
 int dq_pauli_inner(dq_state *bra, dq_state *ket, int n_terms, const uint64_t *xmask,
                    const uint64_t *zmask, const double *coeff, double *re, double *im);
 
+/* Lazy results (no reference counterpart): the k (1..2^20) most probable
+ * basis states, p = |amp|^2 as probabilities() computes it, sorted by
+ * decreasing p, ties by increasing index; the distribution is never copied
+ * to the host (radix select on the device). */
+int dq_top_k(dq_state *s, uint64_t k, uint64_t *idx_out, double *p_out);
+
 /* Compares two states of the same size and any dtypes (the complex64 error
  * report): max_i |a_i - b_i|, <a|b>, and the total-variation distance
  * 0.5 * sum_i | |a_i|^2 - |b_i|^2 |, all accumulated in FP64. */
