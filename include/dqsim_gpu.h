@@ -3,8 +3,8 @@
  *
  * These entry points replace the Rayon kernels of the reference's
  * src/engine.rs one for one; a Rust/PyO3 host binds them through an
- * `extern "C"` block (see INTEGRATION.md for the binding).  Plain pointers
- * and sizes only; no torch or HIP types cross the boundary.
+ * `extern "C"` block.  Plain pointers and sizes only; no torch or HIP types
+ * cross the boundary.
  *
  * State layout in HBM: one contiguous array of 2^n amplitudes, interleaved
  * (re, im), amplitude index bit q <-> qubit q (qubit 0 = LSB), exactly the
