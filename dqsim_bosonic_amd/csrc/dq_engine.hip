@@ -728,7 +728,7 @@ int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *ma
     // level 2 already reorders commuting ops and uses the contracted kernel.
     auto *fn = std::is_same<T, double>::value
                    ? (s->fuse >= 2 ? dq_launch_tile_reg_f64_fma : dq_launch_tile_reg_f64)
-                   : dq_launch_tile_reg_f32;
+                   : (rp.fused_blocks > 0 ? dq_launch_tile_reg_f32_mfma : dq_launch_tile_reg_f32);
     hipError_t e = fn(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles, s->d_partials);
     s->last_grid = int(grid);
     if (e == hipErrorInvalidValue) return fail(DQ_ERR_ARG, "unsupported tile size " + std::to_string(rp.b));

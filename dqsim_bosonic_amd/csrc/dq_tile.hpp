@@ -369,8 +369,10 @@ __device__ __forceinline__ void fused5_mfma(cplx<float> *lds, const cplx<float> 
 // One HBM sweep with register-resident amplitudes (see the file header).
 // The next tile's amplitudes are loaded into registers before the current
 // tile's ops run, so each tile's HBM latency overlaps the previous tile's work.
-// FAST distinguishes the instantiation compiled with contraction (FMA) from
-// the exact one; the source is the same.
+// FAST distinguishes the instantiation compiled with contraction (FMA, 1) from
+// the exact one (0); FAST = 2 is the complex64 contracted instantiation that
+// also carries the MFMA block op (T_FUSED5), kept apart because its
+// accumulators cost registers (occupancy) in sweeps that have no blocks.
 template <typename T, int B, int FAST>
 __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__ amps, TilePass pd,
                                                            const TileOp *__restrict__ ops,
@@ -525,7 +527,7 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 break;
             }
             case T_FUSED5:
-                if constexpr (sizeof(T) == 4 && B >= 12) {
+                if constexpr (sizeof(T) == 4 && B >= 12 && FAST == 2) {  // only the MFMA instantiation
                     __syncthreads();
                     to_lds<T, B>(lds, a, L);
                     __syncthreads();

@@ -1,0 +1,34 @@
+// dq_tile_f32_mfma.hip -- complex64 instantiations of the register-resident tile kernel
+// that include the 5-qubit MFMA block op (fusion level 3); used for sweeps that hold blocks.
+// Compiled with -ffp-contract=fast (the complex64 path makes no bit-exactness claim).
+#include "dq_tile_launch.hpp"
+
+using namespace dq;
+
+namespace {
+
+template <int B>
+hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass &pd, const TileOp *ops,
+                  const void *mats, uint64_t ntiles, double *partials) {
+    using T = float;
+    const size_t shmem = sizeof(cplx<T>) << B;
+    if (shmem > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg<T, B, 2>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_tile_reg<T, B, 2>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
+                       static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles, partials);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t dq_launch_tile_reg_f32_mfma(int b, unsigned grid, hipStream_t stream, void *amps, const TilePass &pd,
+                                       const TileOp *ops, const void *mats, uint64_t ntiles, double *partials) {
+    switch (b) {
+    case 12: return launch<12>(grid, stream, amps, pd, ops, mats, ntiles, partials);
+    case 13: return launch<13>(grid, stream, amps, pd, ops, mats, ntiles, partials);
+    default: return hipErrorInvalidValue;
+    }
+}

@@ -16,3 +16,6 @@ hipError_t dq_launch_tile_reg_f64_fma(int b, unsigned grid, hipStream_t stream, 
 hipError_t dq_launch_tile_reg_f32(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
                                   const dq::TileOp *ops, const void *mats, uint64_t ntiles,
                                   double *partials);
+hipError_t dq_launch_tile_reg_f32_mfma(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
+                                  const dq::TileOp *ops, const void *mats, uint64_t ntiles,
+                                  double *partials);
