@@ -4,7 +4,7 @@ Drop-in for the reference package's statevector surface
 (reference: dqsim/__init__.py:1-27, src/lib.rs:15-29):
 
     from dqsim_bosonic_amd import StatevectorSimulator, SimulationResult, SimulationProfile
-    from dqsim_bosonic_amd import simulate_monolithic
+    from dqsim_bosonic_amd import simulate_monolithic, MpsSimulator
 
 The classes live in the native module ``_core`` (C++ host) and drive the HIP
 engine ``libdqsim_gpu.so`` through the C ABI of include/dqsim_gpu.h.  There is
@@ -26,6 +26,7 @@ except ImportError as _e:  # pragma: no cover - exercised only on broken install
     ) from _e
 
 from ._core import (  # noqa: E402
+    MpsSimulator,
     SimulationProfile,
     SimulationResult,
     StatevectorSimulator,
@@ -37,6 +38,7 @@ from ._core import (  # noqa: E402
 ENGINE_LIBRARY = _os.path.join(_HERE, "libdqsim_gpu.so")
 
 __all__ = [
+    "MpsSimulator",
     "simulate_batch",
     "simulate_monolithic_shots",
     "StatevectorSimulator",

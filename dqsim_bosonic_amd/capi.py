@@ -29,6 +29,8 @@ EXPORTS = (
     "dq_batch_set_observable", "dq_batch_set_workers", "dq_batch_run", "dq_batch_stats",
     "dq_shots_create", "dq_shots_destroy", "dq_shots_add_gate", "dq_shots_add_measure", "dq_shots_add_reset",
     "dq_shots_add_conditional", "dq_shots_words", "dq_shots_run",
+    "dq_mps_create", "dq_mps_destroy", "dq_mps_load", "dq_mps_to_state", "dq_mps_prob_one", "dq_mps_set_cut",
+    "dq_mps_get_cut",
     "dq_last_error",
 )
 
@@ -89,6 +91,14 @@ def lib() -> ctypes.CDLL:
         L.dq_pauli_inner.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp, _dp, _dp]
         L.dq_set_timing.argtypes = [vp, ctypes.c_int]
         L.dq_get_timing.argtypes = [vp, _dp, u64p]
+        L.dq_mps_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.dq_mps_destroy.argtypes = [vp]
+        L.dq_mps_destroy.restype = None
+        L.dq_mps_load.argtypes = [vp, u64p, _dp]
+        L.dq_mps_to_state.argtypes = [vp, vp]
+        L.dq_mps_prob_one.argtypes = [vp, ctypes.c_int, _dp]
+        L.dq_mps_set_cut.argtypes = [vp, ctypes.c_int]
+        L.dq_mps_get_cut.argtypes = [vp]
         L.dq_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -312,3 +322,44 @@ class Shots:
         _check(lib().dq_shots_run(self.h, base_seed, shots, bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                   _d(secs) if timed else None))
         return bits[: shots * words].reshape(shots, words), secs
+
+
+class Mps:
+    """Owning wrapper of a dq_mps contraction context (matrix-product states)."""
+
+    def __init__(self, n: int):
+        h = ctypes.c_void_p()
+        _check(lib().dq_mps_create(n, ctypes.byref(h)))
+        self.h = h
+        self.n = n
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.dq_mps_destroy(self.h)
+            self.h = None
+
+    def load(self, tensors) -> None:
+        """tensors: n arrays of shape (left, 2, right), complex128."""
+        assert len(tensors) == self.n
+        bond = np.array([1] + [int(t.shape[2]) for t in tensors], dtype=np.uint64)
+        for q, t in enumerate(tensors):
+            if t.ndim != 3 or t.shape[1] != 2 or t.shape[0] != int(bond[q]):
+                raise ValueError(f"site {q}: shape {t.shape} does not continue bond {int(bond[q])}")
+        flat = np.concatenate([np.ascontiguousarray(t, dtype=np.complex128).ravel() for t in tensors])
+        self._keep = flat
+        _check(lib().dq_mps_load(self.h, bond.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                 _d(flat.view(np.float64))))
+
+    def to_state(self, state: "State") -> None:
+        _check(lib().dq_mps_to_state(self.h, state.h))
+
+    def prob_one(self, qubit: int) -> float:
+        p = ctypes.c_double()
+        _check(lib().dq_mps_prob_one(self.h, qubit, ctypes.byref(p)))
+        return p.value
+
+    def set_cut(self, cut: int) -> None:
+        _check(lib().dq_mps_set_cut(self.h, cut))
+
+    def cut(self) -> int:
+        return int(lib().dq_mps_get_cut(self.h))

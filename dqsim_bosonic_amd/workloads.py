@@ -177,3 +177,47 @@ def shots_blocks(n: int = 12, depth: int = 20, seed: int = 42) -> C.Circuit:
     for q in range(n):
         ins.append(C.MeasureInstruction(qubit=q, cbit=mid + q))
     return C.circuit(n, ins, mid + n)
+
+
+def mps_generic(n: int, count: int, seed: int, pair_kinds=("cry", "crx", "cu3", "rxx", "cx")) -> C.Circuit:
+    """Circuit for the matrix-product-state simulator: u3 on every qubit,
+    then `count` gates, 40% one-qubit {u3, rx, ry} on a random qubit, else a
+    two-qubit gate from ``pair_kinds`` on a random (not necessarily
+    adjacent) pair; angles U[0.3, 2.8] so no gate is close to the identity
+    or to a Clifford."""
+    rng = np.random.default_rng(seed)
+    ang = lambda: float(rng.uniform(0.3, 2.8))  # noqa: E731
+    ins = [C.instruction("u3", qubit=q, theta=ang(), phi=ang(), lam=ang()) for q in range(n)]
+    for _ in range(count):
+        if rng.random() < 0.4:
+            k = ["u3", "rx", "ry"][int(rng.integers(3))]
+            q = int(rng.integers(n))
+            ins.append(C.instruction("u3", qubit=q, theta=ang(), phi=ang(), lam=ang()) if k == "u3"
+                       else C.instruction(k, qubit=q, theta=ang()))
+            continue
+        k = pair_kinds[int(rng.integers(len(pair_kinds)))]
+        a, b = (int(x) for x in rng.choice(n, size=2, replace=False))
+        if k == "rxx":
+            ins.append(C.instruction(k, a=a, b=b, theta=ang()))
+        elif k == "cu3":
+            ins.append(C.instruction(k, control=a, target=b, theta=ang(), phi=ang(), lam=ang()))
+        elif k in ("crx", "cry"):
+            ins.append(C.instruction(k, control=a, target=b, theta=ang()))
+        else:
+            ins.append(C.instruction(k, control=a, target=b))
+    return C.circuit(n, ins)
+
+
+def random_mps(n: int, chi: int, seed: int = 42) -> list:
+    """Random matrix-product state of n sites: bond q = min(2^q, 2^(n-q), chi),
+    complex normal entries, scaled so that sum |psi|^2 = 1 (the norm is
+    computed by the O(n chi^3) transfer-matrix contraction)."""
+    rng = np.random.default_rng(seed)
+    bond = [min(1 << min(q, n - q), chi) for q in range(n + 1)]
+    ts = [(rng.normal(size=(bond[q], 2, bond[q + 1])) + 1j * rng.normal(size=(bond[q], 2, bond[q + 1])))
+          / np.sqrt(2.0 * bond[q]) for q in range(n)]
+    e = np.ones((1, 1), dtype=np.complex128)
+    for a in ts:
+        e = a[:, 0, :].conj().T @ e @ a[:, 0, :] + a[:, 1, :].conj().T @ e @ a[:, 1, :]
+    s = float(e[0, 0].real) ** (0.5 / n)
+    return [np.ascontiguousarray(a / s) for a in ts]

@@ -220,6 +220,32 @@ int dq_shots_words(const dq_shots *s);
 int dq_shots_run(dq_shots *s, uint64_t base_seed, uint64_t shots, uint64_t *bits_out,
                  double *shot_seconds);
 
+/* ---- matrix-product-state contraction ----
+ * Replaces Mps::to_statevector and the p1 sum of Mps::measure
+ * (src/monolithic/mps/simulator.rs:246-279); the site tensors themselves
+ * stay with the caller (the reference's Mps of :68-74).  bond[0..n] are the
+ * bond dimensions (bond[0] = bond[n] = 1, each <= 65536); site q is a
+ * (bond[q], 2, bond[q+1]) array of complex128, row-major as Tensor::idx
+ * (:49-52), interleaved (re, im); the sites are concatenated in order.
+ * psi(x) = A_0[bit 0 of x] ... A_{n-1}[bit n-1 of x] (qubit q = index bit q). */
+typedef struct dq_mps dq_mps;
+
+int dq_mps_create(int n_sites, dq_mps **out);
+void dq_mps_destroy(dq_mps *m);
+/* Uploads the site tensors (a copy; later changes need another load). */
+int dq_mps_load(dq_mps *m, const uint64_t *bond, const double *tensors);
+/* Dense statevector of the loaded tensors into dst (dst has n_sites qubits;
+ * any queued gates on dst are discarded).  Mps::to_statevector (:246-264). */
+int dq_mps_to_state(dq_mps *m, dq_state *dst);
+/* p1 = sum over x with bit `qubit` set of |psi(x)|^2, without writing the
+ * statevector (the p1 of Mps::measure, :266-272, before its clamp). */
+int dq_mps_prob_one(dq_mps *m, int qubit, double *p1);
+/* Bond at which the chain is cut into the two contracted halves: -1 picks
+ * it from the bond dimensions (default), 0..n forces it (for tests). */
+int dq_mps_set_cut(dq_mps *m, int cut);
+/* The cut the next contraction uses (-1 if nothing is loaded). */
+int dq_mps_get_cut(dq_mps *m);
+
 const char *dq_last_error(void);
 
 #ifdef __cplusplus

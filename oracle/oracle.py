@@ -22,6 +22,12 @@ the reference around them, independently of the product's C++ decoder:
   (simulator.rs:215-322) with ``fuse_circuit`` / ``gate_matrix_1q`` /
   ``matmul2x2`` / ``is_identity_2x2`` (src/types.rs:149-293) and
   ``format_cbits`` (types.rs:43-51).
+* ``mps_to_statevector_loop`` restates ``Mps::to_statevector``
+  (src/monolithic/mps/simulator.rs:246-264) literally; ``mps_contract`` is
+  the same contraction vectorised, ``mps_norm2_transfer`` / ``mps_amplitude``
+  independent closed forms for large cases.  ``OracleMps`` /
+  ``simulate_mps_json`` / ``simulate_mps_shots_json`` restate the MPS
+  simulator (mps/simulator.rs:68-563) with numpy's SVD.
 
 Parity status of each piece is listed in the test modules that pin it.
 """
@@ -498,6 +504,200 @@ def simulate_shots_json(text: str, shots: int, base_seed: int) -> dict:
                 apply_one_qubit(state, n, u, item[1])
             else:
                 sim._run_shot(state, n, item[1], cbits, rng)
+        key = format_cbits(cbits, ncb)
+        counts[key] = counts.get(key, 0) + 1
+    return counts
+
+
+# -- MPS: src/monolithic/mps/simulator.rs --------------------------------------
+
+def mps_to_statevector_loop(tensors) -> np.ndarray:
+    """Mps::to_statevector (mps/simulator.rs:246-264), literally: for every
+    basis index, work = [1]; next[r] += work[l] * A_q[l][bit q][r] over the
+    sites in order; amplitude = work[0].  Pure-Python loops: small n only."""
+    n = len(tensors)
+    out = np.zeros(1 << n, dtype=np.complex128)
+    for basis in range(1 << n):
+        work = [1.0 + 0.0j]
+        for q, a in enumerate(tensors):
+            bit = (basis >> q) & 1
+            nxt = [0.0 + 0.0j] * a.shape[2]
+            for left in range(min(len(work), a.shape[0])):
+                for right in range(a.shape[2]):
+                    nxt[right] += work[left] * a[left, bit, right]
+            work = nxt
+        out[basis] = work[0]
+    return out
+
+
+def mps_contract(tensors) -> np.ndarray:
+    """The same contraction vectorised over basis indices: psi[x][bond] grows
+    one site at a time, the new site's bit becoming index bit q."""
+    psi = np.ones((1, 1), dtype=np.complex128)
+    for a in tensors:
+        psi = np.einsum("xl,lbr->bxr", psi, a).reshape(-1, a.shape[2])
+    return psi[:, 0].copy()
+
+
+def mps_norm2_transfer(tensors, qubit: int | None = None) -> float:
+    """sum_x |psi(x)|^2 (restricted to bit `qubit` = 1 when given) by the
+    transfer-matrix contraction E' = sum_b A[b]^H E A[b], O(n chi^3): a closed
+    form independent of the dense contraction."""
+    e = np.ones((1, 1), dtype=np.complex128)
+    for q, a in enumerate(tensors):
+        bits = (1,) if q == qubit else (0, 1)
+        e = sum(a[:, b, :].conj().T @ e @ a[:, b, :] for b in bits)
+    return float(e[0, 0].real)
+
+
+def mps_amplitude(tensors, x: int) -> complex:
+    """One amplitude as the matrix-chain product A_0[b_0] ... A_{n-1}[b_{n-1}]."""
+    v = np.ones((1, 1), dtype=np.complex128)
+    for q, a in enumerate(tensors):
+        v = v @ a[:, (x >> q) & 1, :]
+    return complex(v[0, 0])
+
+
+class OracleMps:
+    """Mps (mps/simulator.rs:68-298) with numpy's LAPACK SVD (singular values
+    decreasing, as nalgebra's DMatrix::svd returns them)."""
+
+    def __init__(self, n: int, max_bond: int | None = None, threshold: float = 1e-12):
+        self.t = []
+        for _ in range(n):
+            a = np.zeros((1, 2, 1), dtype=np.complex128)
+            a[0, 0, 0] = 1.0
+            self.t.append(a)
+        self.max_bond = max_bond
+        self.thr = threshold
+        self.svd_calls = 0
+        # truncations whose cut falls inside a group of equal singular values:
+        # the kept subspace is then a basis choice of the SVD routine
+        self.degenerate_cuts = 0
+
+    def apply_1q(self, q: int, u: np.ndarray):
+        """apply_1q, :94-110."""
+        self.t[q] = np.einsum("oi,lir->lor", u, self.t[q])
+
+    def apply_2q(self, a: int, b: int, u: np.ndarray):
+        """apply_2q, :112-135: SWAP network, gate at (lo, lo + 1), SWAP back."""
+        if a == b:
+            return
+        lo, hi = min(a, b), max(a, b)
+        sw = gate_matrix("swap")
+        for pos in range(hi - 1, lo, -1):
+            self.apply_adjacent(pos, sw)
+        if a > b:  # reverse_2q_order, :565-579
+            u = u.reshape(2, 2, 2, 2).transpose(1, 0, 3, 2).reshape(4, 4)
+        self.apply_adjacent(lo, u)
+        for pos in range(lo + 1, hi):
+            self.apply_adjacent(pos, sw)
+
+    def apply_adjacent(self, q: int, u: np.ndarray):
+        """apply_adjacent_2q, :137-244: merge, gate, SVD, truncate, rescale."""
+        A, B = self.t[q], self.t[q + 1]
+        L, R = A.shape[0], B.shape[2]
+        p = np.einsum("lik,kjr->lijr", A, B)
+        theta = np.einsum("abij,lijr->labr", u.reshape(2, 2, 2, 2), p).reshape(L * 2, 2 * R)
+        uu, s, vh = np.linalg.svd(theta, full_matrices=False)
+        self.svd_calls += 1
+        keep = max(int(np.sum(s > self.thr)), 1)
+        if self.max_bond is not None:
+            keep = min(keep, max(self.max_bond, 1))
+        if keep < s.size and s[keep] > self.thr and s[keep - 1] - s[keep] <= 1e-9 * s[0]:
+            self.degenerate_cuts += 1
+        kept = max(float(np.sqrt(np.sum(s[:keep] ** 2))), 1e-15)
+        total = float(np.sqrt(np.sum(s ** 2)))
+        discarded = float(np.sum(s[keep:] ** 2))
+        scale = total / kept if discarded > 0.0 else 1.0
+        self.t[q] = uu[:, :keep].reshape(L, 2, keep)
+        self.t[q + 1] = ((s[:keep] * scale)[:, None] * vh[:keep]).reshape(keep, 2, R)
+
+    def measure(self, q: int, rng: "Rng") -> int:
+        """measure, :266-279: p1 from the dense state, clamp, draw, project."""
+        state = mps_contract(self.t)
+        idx = np.arange(state.size)
+        p1 = float(np.sum(np.abs(state[(idx >> q) & 1 == 1]) ** 2))
+        p1 = min(max(p1, 0.0), 1.0)
+        outcome = 1 if rng.gen_f64() < p1 else 0
+        prob = p1 if outcome else 1.0 - p1
+        scale = 1.0 / np.sqrt(prob) if prob > 0.0 else 0.0
+        a = self.t[q] * scale
+        a[:, 1 - outcome, :] = 0.0
+        self.t[q] = a
+        return outcome
+
+
+_MPS_GENERIC = {
+    "remote_link_phi_plus": "remote_link_phi_plus", "remote_epr": "remote_link_phi_plus",
+    "epr": "remote_link_phi_plus", "remote_link_psi_minus": "remote_link_psi_minus",
+    "remote_link_psi_plus": "remote_link_psi_plus", "nonlocal_cz": "cz", "remote_cz": "cz",
+    "remote_cx": "cx",
+}
+
+
+def _mps_run(mps: OracleMps, inst: dict, cbits: dict, rng: "Rng"):
+    """run_instruction of the MPS simulator, mps/simulator.rs:423-563."""
+    kind = inst["kind"]
+    if kind in ("id", "u0", "barrier", "classical"):
+        return
+    if kind == "measure":
+        cbits[inst["cbit"]] = mps.measure(inst["qubit"], rng)
+        return
+    if kind == "reset":
+        if mps.measure(inst["qubit"], rng) == 1:
+            mps.apply_1q(inst["qubit"], gate_matrix("x"))
+        return
+    if kind == "conditional":
+        c = inst["condition"]
+        actual = 0
+        for b in range(c["creg_size"]):
+            actual |= cbits.get(c["creg_base"] + b, 0) << b
+        if actual == c["creg_value"]:
+            _mps_run(mps, inst["op"], cbits, rng)
+        return
+    if kind == "gate":
+        name = inst["name"].lower()
+        if name in ("remote_barrier", "remote_cu1") or name.startswith("circuit-"):
+            return
+        if name not in _MPS_GENERIC:
+            raise NotImplementedError(f"MPS simulator does not support generic gate {name!r}")
+        mps.apply_2q(inst["qubits"][0], inst["qubits"][1], gate_matrix(_MPS_GENERIC[name]))
+        return
+    qubits = [int(inst[f]) for f in _QUBIT_FIELDS[kind]]
+    if len(qubits) > 2:
+        raise NotImplementedError("MPS simulator only supports one- and two-qubit unitary gates plus "
+                                  "measurement, reset, and conditionals")
+    u = gate_matrix(kind, [float(inst[f]) for f in _PARAM_FIELDS.get(kind, ())])
+    if len(qubits) == 1:
+        mps.apply_1q(qubits[0], u)
+    else:
+        mps.apply_2q(qubits[0], qubits[1], u)
+
+
+def simulate_mps_json(text: str, seed: int, max_bond: int | None = None, threshold: float = 1e-12):
+    """MpsSimulator::simulate (mps/simulator.rs:317-345): (statevector, cbits, mps)."""
+    circ = json.loads(text)
+    mps = OracleMps(num_qubits(circ), max_bond, threshold)
+    cbits: dict[int, int] = {}
+    rng = Rng(seed)
+    for inst in circ["instructions"]:
+        _mps_run(mps, inst, cbits, rng)
+    return mps_contract(mps.t), cbits, mps
+
+
+def simulate_mps_shots_json(text: str, shots: int, base_seed: int, max_bond: int | None = None,
+                            threshold: float = 1e-12) -> dict:
+    """MpsSimulator::simulate_shots (mps/simulator.rs:347-421)."""
+    circ = json.loads(text)
+    n, ncb = num_qubits(circ), num_cbits(circ)
+    counts: dict[str, int] = {}
+    for i in range(shots):
+        mps = OracleMps(n, max_bond, threshold)
+        cbits: dict[int, int] = {}
+        rng = Rng((base_seed + i) & (2**64 - 1))
+        for inst in circ["instructions"]:
+            _mps_run(mps, inst, cbits, rng)
         key = format_cbits(cbits, ncb)
         counts[key] = counts.get(key, 0) + 1
     return counts

@@ -133,8 +133,9 @@ def test_statevector_mode_rejects_options():
         dq.simulate_monolithic(object(), mode="state_vector", max_bond_dimension=1)
 
 
-def test_other_algorithms_are_not_this_engine():
-    with pytest.raises(NotImplementedError):
+def test_stabilizer_is_not_this_engine():
+    # mode "mps" is provided (MpsSimulator): it asks the object for its JSON
+    with pytest.raises(AttributeError):
         dq.simulate_monolithic(object(), mode="mps")
     with pytest.raises(NotImplementedError, match="not supported for stabilizer"):
         dq.simulate_monolithic(object(), mode="stabilizer")
