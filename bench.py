@@ -41,6 +41,12 @@ depth ~ U{8..64} (ry, rz on every qubit then a cx ring per layer), <sum_i
 Z_i Z_{i+1}> of each, complex128 and complex64: circuits/s, and the unfused
 algorithmic bytes (every gate reading and writing the whole state) per second.
 
+mps (matrix-product states): Mps::to_statevector of a random 30-site MPS with
+bond <= 16 (upload, growth of both halves, tiled complex GEMM join writing the
+16 GiB statevector) and the p1 of Mps::measure (the same join over half of
+the index space, summed without writing); CPU baseline: the reference's
+per-basis loop restated in C, single-threaded, on a bounded sample.
+
 shots (batched shots): simulate_shots of a 12-qubit random-block circuit with
 mid-circuit measure / reset / conditional and a final measurement of every
 qubit; shots/s on the GPU, and the oracle's sequential restatement on a
@@ -253,6 +259,50 @@ def precision_leg(n: int, depth: int, seed: int, steps: int) -> dict:
     return out
 
 
+def mps_leg(n: int, chi: int, seed: int, reps: int, cpu_seconds: float) -> dict:
+    """Matrix-product-state contraction (Mps::to_statevector and the p1 of
+    Mps::measure) of a random n-site MPS with bond min(2^q, 2^(n-q), chi)."""
+    from dqsim_bosonic_amd import capi
+    from dqsim_bosonic_amd import workloads as W
+
+    ts = W.random_mps(n, chi, seed)
+    m = capi.Mps(n)
+    st = capi.State(n)
+    m.load(ts)
+    m.to_state(st)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.load(ts)  # a fresh upload re-grows both halves: the whole call
+        m.to_state(st)
+    to_state_s = (time.perf_counter() - t0) / reps
+    qs = [0, n // 2, n - 1]
+    t0 = time.perf_counter()
+    for q in qs:
+        m.load(ts)
+        m.prob_one(q)
+    p1_s = (time.perf_counter() - t0) / len(qs)
+    amps = float(1 << n)
+    out = {"workload": f"random MPS {n} sites, bond <= {chi}, cut at bond {m.cut()}",
+           "to_statevector_ms": round(1e3 * to_state_s, 3),
+           "amplitudes_per_s": round(amps / to_state_s, 1),
+           "write_GBps": round(16.0 * amps / to_state_s / 1e9, 1),
+           "fp64_TFLOPs": round(8.0 * chi * amps / to_state_s / 1e12, 2),
+           "measure_p1_ms": round(1e3 * p1_s, 3)}
+    del st, m
+    if cpu_seconds > 0:
+        from oracle import oracle as orc
+
+        done, t0, chunk = 0, time.perf_counter(), 1 << 12
+        while time.perf_counter() - t0 < cpu_seconds:
+            orc.mps_amplitudes_c(ts, done, chunk)
+            done += chunk
+        cpu_el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"amplitudes_per_s": round(done / cpu_el, 1), "cores": 1, "kind": "port",
+                               "sample": f"first {done} basis states through the C restatement of "
+                                         f"Mps::to_statevector's loop ({cpu_el:.1f} s)"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,6 +323,9 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-shots", action="store_true")
+    ap.add_argument("--no-mps", action="store_true")
+    ap.add_argument("--mps-qubits", type=int, default=30)
+    ap.add_argument("--mps-bond", type=int, default=16)
     ap.add_argument("--shots", type=int, default=100000)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-workers", type=int, default=8)
@@ -353,6 +406,12 @@ def main() -> int:
         shots = shots_leg(args.shots, args.seed, 0.0 if args.no_cpu_baseline else 5.0)
         log(f"[bench] shots {shots}")
 
+    mps = None
+    if not args.no_mps:
+        log("[bench] mps contraction ...")
+        mps = mps_leg(args.mps_qubits, args.mps_bond, args.seed, 5, 0.0 if args.no_cpu_baseline else 5.0)
+        log(f"[bench] mps {mps}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -373,6 +432,7 @@ def main() -> int:
         "precision": prec,
         "batch": batch,
         "shots": shots,
+        "mps": mps,
     }
     print(json.dumps(out), flush=True)
     return 0

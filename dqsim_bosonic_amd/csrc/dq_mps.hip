@@ -35,8 +35,11 @@ namespace {
 using C = cplx<double>;
 
 constexpr int kThreads = 256;
-constexpr int kTile = 64;     // join tile: 64 x_lo by 64 x_hi amplitudes per workgroup
-constexpr int kChunk = 16;    // bond entries staged in LDS per step of the join
+constexpr int kColReps = 4;                  // join micro-tile: columns (x_lo) per thread
+constexpr int kRowReps = 4;                  // join micro-tile: rows (x_hi) per thread
+constexpr int kTileCols = 16 * kColReps;     // join tile, x_lo
+constexpr int kTileRows = 16 * kRowReps;     // join tile, x_hi
+constexpr int kChunk = 8;                    // bond entries staged in LDS per step of the join
 constexpr uint64_t kMaxBond = uint64_t(1) << 16;
 constexpr int kMaxSites = 34;
 constexpr uint64_t kMaxWorkBytes = uint64_t(16) << 30;  // L and R growth buffers
@@ -105,13 +108,20 @@ __global__ void k_grow_right(const C *__restrict__ in, C *__restrict__ out, cons
     }
 }
 
+// Statevector stores of the join are non-temporal: the 2^n amplitudes are
+// written once and not read back by this kernel, and keeping them out of the
+// caches keeps L and R there.
 template <typename TO>
 __device__ __forceinline__ void store_amp(cplx<TO> *psi, uint64_t i, C v) {
-    psi[i] = cplx<TO>{TO(v.re), TO(v.im)};
+    typedef TO vec2 __attribute__((ext_vector_type(2)));
+    vec2 x;
+    x.x = TO(v.re);
+    x.y = TO(v.im);
+    __builtin_nontemporal_store(x, reinterpret_cast<vec2 *>(psi + i));
 }
 
-// Tiled join.  Workgroup = one 64 x 64 tile of (row = x_hi, col = x_lo);
-// thread (tx, ty) owns rows ty + 16a and columns tx + 16b (a, b < 4).
+// Tiled join.  Workgroup = one tile of kTileRows rows (x_hi) by kTileCols
+// columns (x_lo); thread (tx, ty) owns rows ty + 16a and columns tx + 16b.
 // Rows / columns are effective indices: with fix_lo (fix_hi) >= 0 the x_lo
 // (x_hi) of column (row) e is e with a 1 inserted at that bit.
 // PROB = false: writes psi[(x_hi << m) | x_lo].  PROB = true: writes the
@@ -121,25 +131,25 @@ __global__ __launch_bounds__(kThreads) void k_join(const C *__restrict__ L, cons
                                                    cplx<TO> *__restrict__ psi, uint32_t chi, int m,
                                                    uint64_t tiles_lo, int fix_lo, int fix_hi,
                                                    double *__restrict__ partials) {
-    __shared__ C Ls[kChunk][kTile];
-    __shared__ C Rs[kTile][kChunk];
+    __shared__ C Ls[kChunk][kTileCols];
+    __shared__ C Rs[kTileRows][kChunk];
     __shared__ double scratch[kThreads / 64];
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     const uint64_t tl = blockIdx.x % tiles_lo, th = blockIdx.x / tiles_lo;
-    const uint64_t col0 = tl * kTile, row0 = th * kTile;
+    const uint64_t col0 = tl * kTileCols, row0 = th * kTileRows;
     const uint64_t wl = uint64_t(1) << m;
-    C acc[4][4];
+    C acc[kRowReps][kColReps];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < kRowReps; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = C{0.0, 0.0};
+        for (int b = 0; b < kColReps; ++b) acc[a][b] = C{0.0, 0.0};
     for (uint32_t c0 = 0; c0 < chi; c0 += kChunk) {
-        for (int e = tid; e < kChunk * kTile; e += kThreads) {
-            const int cc = e / kTile, col = e % kTile;
+        for (int e = tid; e < kChunk * kTileCols; e += kThreads) {
+            const int cc = e / kTileCols, col = e % kTileCols;
             const uint32_t c = c0 + cc;
             Ls[cc][col] = c < chi ? L[uint64_t(c) * wl + with_one(col0 + col, fix_lo)] : C{0.0, 0.0};
         }
-        for (int e = tid; e < kChunk * kTile; e += kThreads) {
+        for (int e = tid; e < kChunk * kTileRows; e += kThreads) {
             const int row = e / kChunk, cc = e % kChunk;
             const uint32_t c = c0 + cc;
             Rs[row][cc] = c < chi ? R[with_one(row0 + row, fix_hi) * chi + c] : C{0.0, 0.0};
@@ -147,37 +157,37 @@ __global__ __launch_bounds__(kThreads) void k_join(const C *__restrict__ L, cons
         __syncthreads();
         const int kc = int(min(uint32_t(kChunk), chi - c0));
         for (int cc = 0; cc < kc; ++cc) {
-            C l[4], r[4];
+            C l[kColReps], r[kRowReps];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) l[b] = Ls[cc][tx + 16 * b];
+            for (int b = 0; b < kColReps; ++b) l[b] = Ls[cc][tx + 16 * b];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) r[a] = Rs[ty + 16 * a][cc];
+            for (int a = 0; a < kRowReps; ++a) r[a] = Rs[ty + 16 * a][cc];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < kRowReps; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = cmac(acc[a][b], l[b], r[a]);
+                for (int b = 0; b < kColReps; ++b) acc[a][b] = cmac(acc[a][b], l[b], r[a]);
         }
         __syncthreads();
     }
     if constexpr (!PROB) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < kRowReps; ++a) {
             const uint64_t xh = row0 + ty + 16 * a;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) store_amp<TO>(psi, (xh << m) | (col0 + tx + 16 * b), acc[a][b]);
+            for (int b = 0; b < kColReps; ++b) store_amp<TO>(psi, (xh << m) | (col0 + tx + 16 * b), acc[a][b]);
         }
     } else {
         double s = 0.0;
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < kRowReps; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) s += acc[a][b].re * acc[a][b].re + acc[a][b].im * acc[a][b].im;
+            for (int b = 0; b < kColReps; ++b) s += acc[a][b].re * acc[a][b].re + acc[a][b].im * acc[a][b].im;
         const double v = block_sum(s, scratch);
         if (tid == 0) partials[blockIdx.x] = v;
     }
 }
 
-// Join for index spaces too small to tile (fewer than 64 rows or columns):
+// Join for index spaces too small to tile (fewer rows or columns than a tile):
 // one thread per amplitude, grid-stride.
 template <typename TO, bool PROB>
 __global__ __launch_bounds__(kThreads) void k_join_small(const C *__restrict__ L, const C *__restrict__ R,
@@ -253,15 +263,18 @@ struct dq_mps {
 namespace {
 
 // Cut bond: minimises the join (2^n chi_m) plus the growth of both halves,
-// keeping >= 128 rows and columns for the tiled join when n >= 14 and the
-// halves within the workspace limit.
+// with both halves within the workspace limit and, when n >= 16, at least
+// two tiles of rows and of columns (so that the half-space join of p1 still
+// tiles).
 int choose_cut(const dq_mps *M) {
     const int n = M->n;
     if (M->cut_req >= 0) return std::min(M->cut_req, n);
     double best = 0.0;
     int bm = -1;
     for (int m = 0; m <= n; ++m) {
-        if (n >= 14 && (m < 7 || n - m < 7)) continue;
+        if (n >= 16 && ((uint64_t(1) << m) < 2 * uint64_t(kTileCols) ||
+                        (uint64_t(1) << (n - m)) < 2 * uint64_t(kTileRows)))
+            continue;
         double cost = std::ldexp(double(M->bond[m]), n), memL = 0.0, memR = 0.0;
         for (int k = 0; k < m; ++k) {
             cost += std::ldexp(double(M->bond[k]) * double(M->bond[k + 1]), k + 1);
@@ -344,8 +357,8 @@ int join(dq_mps *M, cplx<TO> *psi, int fix_lo, int fix_hi, uint64_t *nblocks) {
     const uint32_t chi = uint32_t(M->bond[m]);
     const uint64_t ncols = (uint64_t(1) << m) >> (fix_lo >= 0 ? 1 : 0);
     const uint64_t nrows = (uint64_t(1) << (n - m)) >> (fix_hi >= 0 ? 1 : 0);
-    if (ncols >= uint64_t(kTile) && nrows >= uint64_t(kTile)) {
-        const uint64_t tiles_lo = ncols / kTile, tiles = tiles_lo * (nrows / kTile);
+    if (ncols >= uint64_t(kTileCols) && nrows >= uint64_t(kTileRows)) {
+        const uint64_t tiles_lo = ncols / kTileCols, tiles = tiles_lo * (nrows / kTileRows);
         if (PROB)
             if (int rc = ensure_partials(M, tiles)) return rc;
         hipLaunchKernelGGL((k_join<TO, PROB>), dim3(unsigned(tiles)), dim3(kThreads), 0, M->stream, M->L, M->R,

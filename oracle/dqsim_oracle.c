@@ -510,3 +510,40 @@ int or_gate_matrix(const char *name, const double *p, int np, double *out) {
 #undef NEED
     return -1;
 }
+
+/* Mps::to_statevector, src/monolithic/mps/simulator.rs:246-264, for the basis
+ * indices x_begin .. x_begin + count - 1: per index, work = [1]; for each site
+ * q in order, next[r] = sum over l ascending of work[l] * A_q[l][bit q][r];
+ * amplitude = work[0].  bond[0..n] are the bond dimensions; site q is a
+ * (bond[q], 2, bond[q+1]) row-major complex array, sites concatenated.
+ * Single-threaded, as the reference's loop. */
+void or_mps_amplitudes(int n, const uint64_t *bond, const double *tensors, uint64_t x_begin, uint64_t count,
+                       double *out) {
+    uint64_t maxb = 1, off[65];
+    off[0] = 0;
+    for (int q = 0; q < n; ++q) {
+        if (bond[q + 1] > maxb) maxb = bond[q + 1];
+        off[q + 1] = off[q] + 2 * bond[q] * bond[q + 1];
+    }
+    cx *work = (cx *)malloc(sizeof(cx) * maxb), *next = (cx *)malloc(sizeof(cx) * maxb);
+    for (uint64_t t = 0; t < count; ++t) {
+        const uint64_t x = x_begin + t;
+        uint64_t wl = 1;
+        work[0] = C(1.0, 0.0);
+        for (int q = 0; q < n; ++q) {
+            const uint64_t bit = (x >> q) & 1, L = bond[q], R = bond[q + 1];
+            const double *A = tensors + 2 * off[q];
+            for (uint64_t r = 0; r < R; ++r) next[r] = C(0.0, 0.0);
+            for (uint64_t l = 0; l < wl && l < L; ++l)
+                for (uint64_t r = 0; r < R; ++r)
+                    next[r] = cadd(next[r], cmul(work[l], ld(A, (l * 2 + bit) * R + r)));
+            cx *tmp = work;
+            work = next;
+            next = tmp;
+            wl = R;
+        }
+        st(out, t, work[0]);
+    }
+    free(work);
+    free(next);
+}

@@ -60,6 +60,11 @@ void or_sample_counts(const double *probs, int k, uint64_t shots, or_rng *rng,
 void or_inner(const double *a, const double *b, uint64_t len, double *re,
               double *im);
 
+/* Mps::to_statevector (src/monolithic/mps/simulator.rs:246-264) for the basis
+ * indices x_begin .. x_begin + count - 1 (out: count complex values). */
+void or_mps_amplitudes(int n, const uint64_t *bond, const double *tensors, uint64_t x_begin, uint64_t count,
+                       double *out);
+
 /* OpenMP variants with the reference's Rayon decomposition (CPU baseline). */
 void or_apply_one_qubit_par(double *state, int n, const double u[8], int target);
 void or_apply_n_qubit_par(double *state, int n, const double *u,

@@ -88,6 +88,8 @@ def lib() -> ctypes.CDLL:
                                       ctypes.c_uint64, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_uint32)]
         L.or_num_threads.restype = ctypes.c_int
+        L.or_mps_amplitudes.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), dp, ctypes.c_uint64,
+                                        ctypes.c_uint64, dp]
         L.or_pcg32_output.argtypes = [ctypes.c_uint64]
         L.or_pcg32_output.restype = ctypes.c_uint32
         L.or_rng_from_seed.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
@@ -527,6 +529,17 @@ def mps_to_statevector_loop(tensors) -> np.ndarray:
                     nxt[right] += work[left] * a[left, bit, right]
             work = nxt
         out[basis] = work[0]
+    return out
+
+
+def mps_amplitudes_c(tensors, x_begin: int, count: int) -> np.ndarray:
+    """The literal loop of Mps::to_statevector in C (dqsim_oracle.c) for a
+    range of basis indices: the CPU baseline of the contraction."""
+    bond = np.array([1] + [int(a.shape[2]) for a in tensors], dtype=np.uint64)
+    flat = np.concatenate([np.ascontiguousarray(a, dtype=np.complex128).ravel() for a in tensors])
+    out = np.zeros(count, dtype=np.complex128)
+    lib().or_mps_amplitudes(len(tensors), bond.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                            _dp(flat.view(np.float64)), x_begin, count, _dp(out.view(np.float64)))
     return out
 
 

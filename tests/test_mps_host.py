@@ -212,3 +212,16 @@ def test_oracle_mps_shots_full_bond_equal_statevector_shots():
     text = C.circuit(4, ins, n_cbits=4).model_dump_json()
     assert orc.simulate_mps_shots_json(text, 60, 42, max_bond=4) == orc.simulate_shots_json(text, 60, 42)
     json.loads(text)
+
+
+def test_c_loop_equals_python_loop_bitwise():
+    # the C restatement (the contraction's CPU baseline) and the Python one
+    # follow the same op order, so they agree bit for bit
+    rng = np.random.default_rng(12)
+    for bonds in ([1, 2, 1], [1, 2, 4, 3, 2, 1], [1, 2, 4, 8, 4, 2, 1]):
+        ts = rand_mps(rng, bonds)
+        n = len(ts)
+        want = orc.mps_to_statevector_loop(ts)
+        assert np.array_equal(orc.mps_amplitudes_c(ts, 0, 1 << n), want)
+        cnt = min(5, (1 << n) - 3)
+        assert np.array_equal(orc.mps_amplitudes_c(ts, 3, cnt), want[3:3 + cnt])
