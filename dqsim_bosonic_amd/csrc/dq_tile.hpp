@@ -599,15 +599,19 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
         for (tile += gridDim.x; tile < ntiles; tile += gridDim.x) process(tile);
     }
     if (pd.red_bit >= 0) {
-        // fixed-shape block sum: wave butterflies, then wave totals in order
-        for (int off = 32; off > 0; off >>= 1) red += __shfl_xor(red, off, 64);
+        // fixed-shape block sum: butterflies within each group of kGroup
+        // lanes (a wave, or the whole block when it is smaller than a wave:
+        // 32 threads at B = 9), then the group totals in order
+        constexpr int kThr = 1 << (B - 4);
+        constexpr int kGroup = kThr < 64 ? kThr : 64;
+        for (int off = kGroup / 2; off > 0; off >>= 1) red += __shfl_xor(red, off, kGroup);
         __syncthreads();
         double *scratch = reinterpret_cast<double *>(smem_raw);
-        if ((t & 63) == 0) scratch[t >> 6] = red;
+        if ((t % kGroup) == 0) scratch[t / kGroup] = red;
         __syncthreads();
         if (t == 0) {
             double r = 0.0;
-            for (int w = 0; w < (1 << (B - 4)) / 64; ++w) r += scratch[w];
+            for (int w = 0; w < kThr / kGroup; ++w) r += scratch[w];
             partials[blockIdx.x] = r;
         }
     }

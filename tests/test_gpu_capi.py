@@ -283,3 +283,30 @@ def test_prob_one_fused_into_sweep(fuse):
         p = st.prob_one(q)
         assert abs(p - orc.prob_one(ref, n, q)) <= 1e-14
         st.close()
+
+
+@pytest.mark.parametrize("fuse", [0, 1, 2])
+def test_prob_one_fused_into_sweep_every_width(fuse):
+    """Regression: the reduction summed by the last sweep of a flush, at
+    every state width (tiles of 9 qubits run 32-thread workgroups, half a
+    wave, at n = 9..11)."""
+    for n in range(1, 17):
+        rng = np.random.default_rng(300 + n)
+        psi = random_state(n, rng)
+        prog = random_program(n, 12, rng)
+        for q in sorted({0, n // 2, n - 1}):
+            st = capi.State(n)
+            st.set_options(fuse, 0)
+            st.set(psi)
+            ref = psi.copy()
+            for item in prog:
+                if item[1] is not None:
+                    u = orc.gate_matrix(item[0], item[1])
+                    st.apply_1q(u, item[2][0])
+                    orc.apply_one_qubit(ref, n, u, item[2][0])
+                else:
+                    st.apply_nq(item[3], item[2])
+                    orc.apply_n_qubit(ref, n, item[3], item[2])
+            p = st.prob_one(q)
+            assert abs(p - orc.prob_one(ref, n, q)) <= 1e-14, (n, q)
+            st.close()

@@ -104,6 +104,17 @@ def test_measurement_record_matches(seed):
     assert err <= 1e-12, err
 
 
+@pytest.mark.parametrize("n", [9, 10, 11, 12])
+@pytest.mark.parametrize("fusion", [1, 2])
+def test_measurement_record_every_small_tile_width(n, fusion):
+    """Regression: widths whose sweeps use 9-qubit tiles (n = 9..11) measure
+    with the reduction fused into the last sweep, like the wider ones."""
+    circ = W.random_blocks(n, 16, seed=n, remote_per_kind=2, measure_every=4)
+    r, o = both(circ, n, fusion)
+    assert r.classical_bits == o.classical_bits
+    assert np.max(np.abs(r.statevector - o.statevector)) <= 1e-12
+
+
 def test_fidelity_and_length_mismatch():
     circ = W.brickwork_u3(10, 6, seed=3)
     r, o = both(circ)
