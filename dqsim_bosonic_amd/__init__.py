@@ -27,10 +27,14 @@ except ImportError as _e:  # pragma: no cover - exercised only on broken install
 
 from ._core import (  # noqa: E402
     MpsSimulator,
+    PBlockResult,
+    PBlockSimulator,
     SimulationProfile,
     SimulationResult,
     StatevectorSimulator,
     simulate_batch,
+    simulate_distributed,
+    simulate_distributed_shots,
     simulate_monolithic,
     simulate_monolithic_shots,
 )
@@ -39,6 +43,10 @@ ENGINE_LIBRARY = _os.path.join(_HERE, "libdqsim_gpu.so")
 
 __all__ = [
     "MpsSimulator",
+    "PBlockResult",
+    "PBlockSimulator",
+    "simulate_distributed",
+    "simulate_distributed_shots",
     "simulate_batch",
     "simulate_monolithic_shots",
     "StatevectorSimulator",

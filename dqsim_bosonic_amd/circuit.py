@@ -37,6 +37,8 @@ _KIND_FIELDS: dict[str, tuple[str, ...]] = {
     "barrier": (),
     "classical": ("name",),
 }
+_QUBIT_FIELDS = ("qubit", "control", "target", "a", "b", "control1", "control2", "control3", "control4",
+                 "target1", "target2")
 
 
 class Register:
@@ -121,3 +123,41 @@ class JsonCircuit:
 
     def model_dump_json(self) -> str:
         return self.text
+
+
+class DistributedCircuit:
+    """The attributes of bosonic_model's DistributedCircuit that the P-block
+    simulator reads (src/distributed/pblock/simulator.rs:286-348):
+    ``qubits_per_node`` {node: [physical qubits]}, ``circuits`` {node:
+    Circuit} whose instruction lists may share instruction objects (a gate
+    across nodes appears in each of its nodes), and ``_instruction_index``
+    {id(instruction object): global order}."""
+
+    def __init__(self, qubits_per_node: dict, circuits: dict, instruction_index: dict | None = None):
+        self.qubits_per_node = qubits_per_node
+        self.circuits = circuits
+        self._instruction_index = dict(instruction_index or {})
+
+
+def distribute(circ: Circuit, qubits_per_node: dict) -> DistributedCircuit:
+    """Splits a monolithic circuit by node: each instruction object goes to
+    every node owning one of its qubits (the same object, so it is simulated
+    once), in program order; barriers and classical instructions go to the
+    lowest node.  Registers are kept whole in every node circuit."""
+    owner = {q: node for node, qs in qubits_per_node.items() for q in qs}
+    lists: dict = {node: [] for node in qubits_per_node}
+    index = {}
+    first = min(qubits_per_node)
+    for order, ins in enumerate(circ.instructions):
+        d = ins.model_dump()
+        if d["kind"] == "conditional":
+            d = d["op"]
+        qs = [d[k] for k in _KIND_FIELDS[d["kind"]] if k in _QUBIT_FIELDS]
+        if d["kind"] == "gate":
+            qs = list(d["qubits"])
+        nodes = sorted({owner[q] for q in qs}) or [first]
+        for node in nodes:
+            lists[node].append(ins)
+        index[id(ins)] = order
+    circuits = {node: Circuit(circ.qregs, circ.cregs, lists[node]) for node in qubits_per_node}
+    return DistributedCircuit(qubits_per_node, circuits, index)

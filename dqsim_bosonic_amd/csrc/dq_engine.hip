@@ -358,6 +358,27 @@ __global__ __launch_bounds__(kThreads) void k_copy_state(cplx<T> *__restrict__ d
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = src[i];
 }
 
+// Tensor product with interleaved qubits (merge_blocks of the reference's
+// pblock model, src/distributed/pblock/model.rs:30-58):
+// dst[i] = a[bits of i at apos] * b[bits of i at bpos], one complex product
+// in num-complex order per amplitude.
+struct TensorPos {
+    int na, nb;
+    int apos[40], bpos[40];
+};
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_tensor(cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ a,
+                                                     const cplx<T> *__restrict__ b, TensorPos tp, uint64_t count) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        uint64_t ia = 0, ib = 0;
+        for (int k = 0; k < tp.na; ++k) ia |= ((i >> tp.apos[k]) & 1) << k;
+        for (int k = 0; k < tp.nb; ++k) ib |= ((i >> tp.bpos[k]) & 1) << k;
+        dst[i] = cmul(a[ia], b[ib]);
+    }
+}
+
 // Differences between two states of the same size (any dtypes): partials of
 // max |a_i - b_i|, Re/Im of sum conj(a_i) b_i and sum | |a_i|^2 - |b_i|^2 |.
 template <typename TA, typename TB>
@@ -1271,6 +1292,41 @@ int dq_state_copy(dq_state *dst, dq_state *src) {
     HIP_TRY(hipGetLastError());
     dst->launches++;
     // src may be modified on its own stream right after this call returns
+    HIP_TRY(hipStreamSynchronize(dst->stream));
+    return DQ_OK;
+}
+
+int dq_state_tensor(dq_state *dst, dq_state *a, const int *a_pos, dq_state *b, const int *b_pos) {
+    if (!dst || !a || !b || (a->n && !a_pos) || (b->n && !b_pos)) return fail(DQ_ERR_ARG, "null argument");
+    if (dst == a || dst == b) return fail(DQ_ERR_ARG, "dq_state_tensor needs a destination distinct from its factors");
+    if (dst->n != a->n + b->n) return fail(DQ_ERR_ARG, "destination must have the factors' qubit count combined");
+    if (dst->dtype != a->dtype || dst->dtype != b->dtype) return fail(DQ_ERR_ARG, "states differ in dtype");
+    TensorPos tp;
+    tp.na = a->n;
+    tp.nb = b->n;
+    uint64_t seen = 0;
+    for (int k = 0; k < a->n + b->n; ++k) {
+        const int p = k < a->n ? a_pos[k] : b_pos[k - a->n];
+        if (p < 0 || p >= dst->n || ((seen >> p) & 1))
+            return fail(DQ_ERR_ARG, "a_pos and b_pos must name every destination qubit once");
+        seen |= uint64_t(1) << p;
+        (k < a->n ? tp.apos[k] : tp.bpos[k - a->n]) = p;
+    }
+    if (int rc = join(dst, a)) return rc;
+    if (int rc = join(dst, b)) return rc;
+    dst->queue.clear();
+    const uint64_t count = amp_count(dst);
+    if (dst->dtype == DQ_C128)
+        hipLaunchKernelGGL(k_tensor<double>, dim3(grid_for(count)), dim3(kThreads), 0, dst->stream,
+                           static_cast<cplx<double> *>(dst->amps), static_cast<const cplx<double> *>(a->amps),
+                           static_cast<const cplx<double> *>(b->amps), tp, count);
+    else
+        hipLaunchKernelGGL(k_tensor<float>, dim3(grid_for(count)), dim3(kThreads), 0, dst->stream,
+                           static_cast<cplx<float> *>(dst->amps), static_cast<const cplx<float> *>(a->amps),
+                           static_cast<const cplx<float> *>(b->amps), tp, count);
+    HIP_TRY(hipGetLastError());
+    dst->launches++;
+    // a and b may be modified or freed right after this call returns
     HIP_TRY(hipStreamSynchronize(dst->stream));
     return DQ_OK;
 }

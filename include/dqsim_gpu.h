@@ -98,6 +98,13 @@ int dq_copy_to_host(dq_state *s, double *dst);
 /* dst = src (amplitudes; both states must have the same size and dtype). */
 int dq_state_copy(dq_state *dst, dq_state *src);
 
+/* dst = a (x) b with the qubits interleaved: local qubit k of a becomes
+ * qubit a_pos[k] of dst, local qubit k of b qubit b_pos[k]; together they
+ * name every qubit of dst (dst->n = a->n + b->n) once.  Each amplitude is
+ * one complex product, as merge_blocks (src/distributed/pblock/model.rs:30-58)
+ * computes it.  dst must be distinct from a and b; all three share a dtype. */
+int dq_state_tensor(dq_state *dst, dq_state *a, const int *a_pos, dq_state *b, const int *b_pos);
+
 /* dst = sum_k coeff_k P_k src  (dst and src distinct). */
 int dq_apply_pauli_sum(dq_state *dst, dq_state *src, int n_terms, const uint64_t *xmask,
                        const uint64_t *zmask, const double *coeff);

@@ -714,3 +714,248 @@ def simulate_mps_shots_json(text: str, shots: int, base_seed: int, max_bond: int
         key = format_cbits(cbits, ncb)
         counts[key] = counts.get(key, 0) + 1
     return counts
+
+
+# -- P-block: src/distributed/pblock --------------------------------------------
+
+def merge_blocks_loop(a_state, a_qubits, b_state, b_qubits):
+    """merge_blocks (pblock/model.rs:30-58) literally: the product of the two
+    factors' amplitudes over the sorted union of their qubits."""
+    qubits = sorted(set(a_qubits) | set(b_qubits))
+    out = np.zeros(1 << len(qubits), dtype=np.complex128)
+    for i in range(out.size):
+        al = 0
+        for bit, q in enumerate(a_qubits):
+            if (i >> qubits.index(q)) & 1:
+                al |= 1 << bit
+        bl = 0
+        for bit, q in enumerate(b_qubits):
+            if (i >> qubits.index(q)) & 1:
+                bl |= 1 << bit
+        x, y = complex(a_state[al]), complex(b_state[bl])
+        out[i] = complex(x.real * y.real - x.imag * y.imag, x.real * y.imag + x.imag * y.real)
+    return out, qubits
+
+
+def merge_blocks(a_state, a_qubits, b_state, b_qubits):
+    """merge_blocks vectorised over the merged index."""
+    qubits = sorted(set(a_qubits) | set(b_qubits))
+    idx = np.arange(1 << len(qubits), dtype=np.int64)
+    al = np.zeros_like(idx)
+    bl = np.zeros_like(idx)
+    for bit, q in enumerate(a_qubits):
+        al |= ((idx >> qubits.index(q)) & 1) << bit
+    for bit, q in enumerate(b_qubits):
+        bl |= ((idx >> qubits.index(q)) & 1) << bit
+    x, y = a_state[al], b_state[bl]
+    out = np.empty(idx.size, dtype=np.complex128)  # num-complex product, one rounding per operation
+    out.real = x.real * y.real - x.imag * y.imag
+    out.imag = x.real * y.imag + x.imag * y.real
+    return out, qubits
+
+
+class OracleBlockPool:
+    """BlockPool (pblock/model.rs:60-132)."""
+
+    def __init__(self, qpn: dict):
+        self.blocks = []
+        self.q2b = {}
+        self.merge_count = 0
+        for node in sorted(qpn):
+            qs = sorted(set(qpn[node]))
+            for q in qpn[node]:
+                self.q2b[q] = len(self.blocks)
+            st = np.zeros(1 << len(qs), dtype=np.complex128)
+            st[0] = 1.0
+            self.blocks.append([st, qs])
+
+    def block_of(self, q):
+        if q not in self.q2b:
+            raise ValueError(f"qubit {q} is not assigned to any block")
+        return self.q2b[q]
+
+    def merge(self, keep, drop):
+        if keep == drop:
+            return
+        a, b = self.blocks[keep], self.blocks[drop]
+        self.blocks[drop] = None
+        for q in b[1]:
+            self.q2b[q] = keep
+        self.blocks[keep] = list(merge_blocks(a[0], a[1], b[0], b[1]))
+        self.merge_count += 1
+
+    def ensure_single_block(self, qubits):
+        if not qubits:
+            return 0
+        target = self.block_of(qubits[0])
+        for q in qubits[1:]:
+            bq = self.block_of(q)
+            if bq != target:
+                self.merge(target, bq)
+        return target
+
+    def merge_all(self):
+        acc = None
+        for b in self.blocks:
+            if b is None:
+                continue
+            acc = b if acc is None else list(merge_blocks(acc[0], acc[1], b[0], b[1]))
+        if acc is None:
+            return [np.ones(1, dtype=np.complex128), []]
+        return acc
+
+
+def pblock_read(distributed):
+    """The entry stream of pblock/simulator.rs:286-348: (order, node,
+    local index) per distinct instruction object (Python identity), ordered by
+    ``_instruction_index`` (unknown objects after the known ones, in encounter
+    order); node circuits as JSON dicts; the largest cbit count."""
+    index = dict(distributed._instruction_index)
+    seen, entries, fallback, circuits = set(), [], (2**63 - 1) // 2, {}
+    for node in sorted(distributed.circuits):
+        c = distributed.circuits[node]
+        circuits[node] = json.loads(c.model_dump_json())
+        for i, inst in enumerate(c.instructions):
+            if id(inst) in seen:
+                continue
+            seen.add(id(inst))
+            if id(inst) in index:
+                order = index[id(inst)]
+            else:
+                order, fallback = fallback, fallback + 1
+            entries.append((order, node, i))
+    entries.sort(key=lambda e: e[0])
+    ncb = max((num_cbits(c) for c in circuits.values()), default=0)
+    return entries, circuits, dict(distributed.qubits_per_node), ncb
+
+
+_PBLOCK_GENERIC = {
+    "remote_link_phi_plus": "remote_link_phi_plus", "remote_link_psi_minus": "remote_link_psi_minus",
+    "remote_link_psi_plus": "remote_link_psi_plus", "nonlocal_cz": "cz", "remote_cz": "cz",
+    "remote_cx": "cx", "remote_epr": "remote_link_phi_plus",
+}
+
+
+def _pblock_dispatch(block, inst, cbits, rng):
+    """dispatch / dispatch_par (pblock/simulator.rs:463-710, :746-985) on one block."""
+    state, qubits = block
+    n = len(qubits)
+    kind = inst["kind"]
+    if kind in ("id", "u0", "barrier", "classical"):
+        return
+    if kind in ("measure", "reset"):
+        lq = qubits.index(inst["qubit"])
+        d = _dp(state.view(np.float64))
+        p1 = float(lib().or_prob_one(d, n, lq))
+        outcome = 1 if rng.gen_f64() < p1 else 0
+        lib().or_collapse(d, n, lq, outcome, p1)
+        if kind == "measure":
+            cbits[inst["cbit"]] = outcome
+        elif outcome == 1:
+            apply_one_qubit(state, n, gate_matrix("x"), lq)
+        return
+    if kind == "conditional":
+        c = inst["condition"]
+        actual = 0
+        for b in range(c["creg_size"]):
+            actual |= cbits.get(c["creg_base"] + b, 0) << b
+        if actual == c["creg_value"]:
+            _pblock_dispatch(block, inst["op"], cbits, rng)
+        return
+    if kind == "gate":
+        name = inst["name"].lower()
+        lq = [qubits.index(q) for q in inst["qubits"]]
+        if name in ("remote_barrier", "remote_cu1") or name.startswith("circuit-"):
+            return
+        if name == "remote_rzz":
+            u = gate_matrix("rzz", [float(inst["params"][0])])
+        elif name == "teleport":
+            raise NotImplementedError("Symbolic 'teleport' gate cannot be simulated natively.")
+        elif name in _PBLOCK_GENERIC:
+            u = gate_matrix(_PBLOCK_GENERIC[name])
+        else:
+            raise NotImplementedError(f"Unsupported gate in pblock simulator: {name!r}")
+        apply_n_qubit(state, n, u, lq)
+        return
+    lq = [qubits.index(int(inst[f])) for f in _QUBIT_FIELDS[kind]]
+    u = gate_matrix(kind, [float(inst[f]) for f in _PARAM_FIELDS.get(kind, ())])
+    if len(lq) == 1:
+        apply_one_qubit(state, n, u, lq[0])
+    else:
+        apply_n_qubit(state, n, u, lq)
+
+
+def simulate_pblock(distributed, seed: int):
+    """PBlockSimulator::simulate (pblock/simulator.rs:285-460), epochs included:
+    returns (statevector, physical qubits, cbits)."""
+    entries, circuits, qpn, _ = pblock_read(distributed)
+    pool = OracleBlockPool(qpn)
+    cbits: dict[int, int] = {}
+    rng = Rng(seed)
+    inst_of = lambda e: circuits[e[1]]["instructions"][e[2]]  # noqa: E731
+    cursor = 0
+    while cursor < len(entries):
+        epoch, used, j = [], set(), cursor
+        while j < len(entries):
+            qs = inst_qubits(inst_of(entries[j]))
+            if not qs:
+                j += 1
+                continue
+            touched = {pool.q2b[q] for q in qs if q in pool.q2b}
+            if len(touched) != 1:
+                break
+            blk = next(iter(touched))
+            if blk in used:
+                break
+            epoch.append((j, blk))
+            used.add(blk)
+            j += 1
+        if len(epoch) > 1:
+            seeds = [rng.next_u64() for _ in epoch]
+            local = [{} for _ in epoch]
+            for k, (ej, blk) in enumerate(epoch):
+                _pblock_dispatch(pool.blocks[blk], inst_of(entries[ej]), local[k], Rng(seeds[k]))
+            for loc in local:
+                cbits.update(loc)
+            cursor = j
+        else:
+            inst = inst_of(entries[cursor])
+            qs = inst_qubits(inst)
+            if qs:
+                blk = pool.ensure_single_block(qs)
+                _pblock_dispatch(pool.blocks[blk], inst, cbits, rng)
+            cursor += 1
+    st, qs = pool.merge_all()
+    return st, qs, cbits
+
+
+def simulate_pblock_shots(distributed, shots: int, base_seed: int):
+    """PBlockSimulator::simulate_shots (pblock/simulator.rs:128-283) with
+    run_shot_entries (:712-744) over fuse_pblock_entries (src/types.rs:306-345)
+    -- the fusion of the globally ordered stream: returns (counts, merges per shot)."""
+    entries, circuits, qpn, ncb = pblock_read(distributed)
+    stream = [circuits[node]["instructions"][i] for _, node, i in entries]
+    fused = fuse_circuit(stream)
+    counts: dict[str, int] = {}
+    merges = 0
+    for i in range(shots):
+        pool = OracleBlockPool(qpn)
+        cbits: dict[int, int] = {}
+        rng = Rng((base_seed + i) & (2**64 - 1))
+        for item in fused:
+            if item[0] == "fused":
+                blk = pool.ensure_single_block([item[1]])
+                st, qs = pool.blocks[blk]
+                u = np.array([complex(*e) for e in item[2]], dtype=np.complex128).reshape(2, 2)
+                apply_one_qubit(st, len(qs), u, qs.index(item[1]))
+                continue
+            inst = item[1]
+            qs = inst_qubits(inst)
+            if not qs:
+                continue
+            blk = pool.ensure_single_block(qs)
+            _pblock_dispatch(pool.blocks[blk], inst, cbits, rng)
+        key = format_cbits(cbits, ncb)
+        counts[key] = counts.get(key, 0) + 1
+        merges = pool.merge_count
+    return counts, merges
