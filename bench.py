@@ -47,6 +47,11 @@ bond <= 16 (upload, growth of both halves, tiled complex GEMM join writing the
 the index space, summed without writing); CPU baseline: the reference's
 per-basis loop restated in C, single-threaded, on a bounded sample.
 
+pblock (distributed circuits): PBlockSimulator.simulate of a 4-node x 7-qubit
+layered circuit (u3 layers, cx chains inside nodes, remote_cx and nonlocal_cz
+across nodes, measure + conditional every 5 layers); instructions/s, CPU
+baseline the oracle's restatement on a bounded prefix.
+
 shots (batched shots): simulate_shots of a 12-qubit random-block circuit with
 mid-circuit measure / reset / conditional and a final measurement of every
 qubit; shots/s on the GPU, and the oracle's sequential restatement on a
@@ -303,6 +308,45 @@ def mps_leg(n: int, chi: int, seed: int, reps: int, cpu_seconds: float) -> dict:
     return out
 
 
+def pblock_leg(nodes: int, per_node: int, depth: int, seed: int, cpu_seconds: float) -> dict:
+    """PBlockSimulator.simulate of a distributed layered circuit: blocks per
+    node merged on demand on the device (dq_state_tensor), epochs as the
+    reference forms them."""
+    from dqsim_bosonic_amd import PBlockSimulator
+    from dqsim_bosonic_amd import workloads as W
+
+    circ, dist = W.distributed_layers(nodes, per_node, depth, seed)
+    _, small = W.distributed_layers(2, 3, 2, seed)
+    PBlockSimulator(seed=seed).simulate(small)  # warm-up
+    t0 = time.perf_counter()
+    res = PBlockSimulator(seed=seed).simulate(dist)
+    el = time.perf_counter() - t0
+    count = len(circ.instructions)
+    out = {"workload": f"distributed_layers {nodes} nodes x {per_node} qubits, depth {depth} "
+                       f"({count} instructions, merged to {res.num_qubits} qubits)",
+           "seconds": round(el, 4), "instructions_per_s": round(count / el, 1)}
+    if cpu_seconds > 0:
+        from oracle import oracle as orc
+
+        # the oracle's 28-qubit merges alone take tens of seconds (numpy index
+        # arithmetic over 2^28 entries), so its sample is the same family at
+        # 4 x 5 qubits, timed on a bounded prefix, and the GPU is timed on it too
+        c20, d20 = W.distributed_layers(nodes, 5, depth, seed)
+        t0 = time.perf_counter()
+        PBlockSimulator(seed=seed).simulate(d20)
+        gpu20 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, _, done = orc._pblock_run(d20, seed, budget_s=cpu_seconds)
+        cpu_el = time.perf_counter() - t0
+        out["gpu_20q_instructions_per_s"] = round(len(c20.instructions) / gpu20, 1)
+        out["cpu_baseline"] = {"instructions_per_s": round(done / cpu_el, 2), "cores": 1, "kind": "port",
+                               "sample": f"distributed_layers {nodes} x 5 qubits (20 qubits): first {done} of "
+                                         f"{len(c20.instructions)} instructions through the oracle's restatement "
+                                         f"of PBlockSimulator::simulate ({cpu_el:.1f} s); compare "
+                                         f"gpu_20q_instructions_per_s"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +368,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-shots", action="store_true")
     ap.add_argument("--no-mps", action="store_true")
+    ap.add_argument("--no-pblock", action="store_true")
     ap.add_argument("--mps-qubits", type=int, default=30)
     ap.add_argument("--mps-bond", type=int, default=16)
     ap.add_argument("--shots", type=int, default=100000)
@@ -412,6 +457,12 @@ def main() -> int:
         mps = mps_leg(args.mps_qubits, args.mps_bond, args.seed, 5, 0.0 if args.no_cpu_baseline else 5.0)
         log(f"[bench] mps {mps}")
 
+    pblock = None
+    if not args.no_pblock:
+        log("[bench] pblock ...")
+        pblock = pblock_leg(4, 7, 20, args.seed, 0.0 if args.no_cpu_baseline else 5.0)
+        log(f"[bench] pblock {pblock}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -433,6 +484,7 @@ def main() -> int:
         "batch": batch,
         "shots": shots,
         "mps": mps,
+        "pblock": pblock,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -221,3 +221,34 @@ def random_mps(n: int, chi: int, seed: int = 42) -> list:
         e = a[:, 0, :].conj().T @ e @ a[:, 0, :] + a[:, 1, :].conj().T @ e @ a[:, 1, :]
     s = float(e[0, 0].real) ** (0.5 / n)
     return [np.ascontiguousarray(a / s) for a in ts]
+
+
+def distributed_layers(nodes: int = 4, per_node: int = 7, depth: int = 20, seed: int = 42):
+    """Distributed circuit for the P-block simulator: per layer u3 on every
+    qubit, a cx chain inside every node, one remote_cx and one nonlocal_cz
+    between random nodes; after every 5th layer a measurement of a random
+    qubit and an ry conditioned on it.  Returns (monolithic circuit,
+    DistributedCircuit) with the nodes owning consecutive qubits."""
+    rng = np.random.default_rng(seed)
+    n = nodes * per_node
+    ins = []
+    for layer in range(depth):
+        for q in range(n):
+            ins.append(C.instruction("u3", qubit=q, theta=float(rng.uniform(0, TWO_PI)),
+                                     phi=float(rng.uniform(0, TWO_PI)), lam=float(rng.uniform(0, TWO_PI))))
+        for k in range(nodes):
+            for q in range(k * per_node, (k + 1) * per_node - 1):
+                ins.append(C.instruction("cx", control=q, target=q + 1))
+        for name in ("remote_cx", "nonlocal_cz"):
+            a, b = (int(x) for x in rng.choice(nodes, size=2, replace=False))
+            qa = a * per_node + int(rng.integers(per_node))
+            qb = b * per_node + int(rng.integers(per_node))
+            ins.append(C.GateInstruction(name=name, qubits=[qa, qb], params=[]))
+        if layer % 5 == 4:
+            q = int(rng.integers(n))
+            ins.append(C.instruction("measure", qubit=q, cbit=layer // 5))
+            ins.append(C.instruction("conditional", condition=C.Condition(layer // 5, 1, 1),
+                                     op=C.instruction("ry", qubit=int(rng.integers(n)), theta=0.9)))
+    circ = C.circuit(n, ins, n_cbits=max(1, depth // 5))
+    qpn = {k: list(range(k * per_node, (k + 1) * per_node)) for k in range(nodes)}
+    return circ, C.distribute(circ, qpn)

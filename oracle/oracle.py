@@ -885,16 +885,22 @@ def _pblock_dispatch(block, inst, cbits, rng):
         apply_n_qubit(state, n, u, lq)
 
 
-def simulate_pblock(distributed, seed: int):
-    """PBlockSimulator::simulate (pblock/simulator.rs:285-460), epochs included:
-    returns (statevector, physical qubits, cbits)."""
+def _pblock_run(distributed, seed: int, budget_s: float | None = None):
+    """The instruction loop of PBlockSimulator::simulate (pblock/simulator.rs:
+    368-443): returns (pool, cbits, entries done); with budget_s it stops at
+    the first epoch boundary after that many seconds."""
+    import time
+
     entries, circuits, qpn, _ = pblock_read(distributed)
     pool = OracleBlockPool(qpn)
     cbits: dict[int, int] = {}
     rng = Rng(seed)
     inst_of = lambda e: circuits[e[1]]["instructions"][e[2]]  # noqa: E731
     cursor = 0
+    t0 = time.perf_counter()
     while cursor < len(entries):
+        if budget_s is not None and time.perf_counter() - t0 >= budget_s:
+            break
         epoch, used, j = [], set(), cursor
         while j < len(entries):
             qs = inst_qubits(inst_of(entries[j]))
@@ -925,6 +931,13 @@ def simulate_pblock(distributed, seed: int):
                 blk = pool.ensure_single_block(qs)
                 _pblock_dispatch(pool.blocks[blk], inst, cbits, rng)
             cursor += 1
+    return pool, cbits, cursor
+
+
+def simulate_pblock(distributed, seed: int):
+    """PBlockSimulator::simulate (pblock/simulator.rs:285-460), epochs included:
+    returns (statevector, physical qubits, cbits)."""
+    pool, cbits, _ = _pblock_run(distributed, seed)
     st, qs = pool.merge_all()
     return st, qs, cbits
 
