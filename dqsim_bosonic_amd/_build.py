@@ -49,7 +49,8 @@ def core_path() -> str:
 # separately; the contracted ones may fuse them into FMAs.
 ENGINE_SOURCES = (("dq_engine.hip", "off"), ("dq_tile_f64.hip", "off"), ("dq_tile_f64_fma.hip", "fast"),
                   ("dq_tile_f32.hip", "fast"), ("dq_tile_f32_mfma.hip", "fast"), ("dq_batch.hip", "off"),
-                  ("dq_shots.hip", "off"), ("dq_mps.hip", "fast"))
+                  ("dq_shots.hip", "off"), ("dq_mps.hip", "fast"),
+                  ("dq_adjoint.hip", "off"))
 
 
 def build_engine() -> str:

@@ -22,7 +22,7 @@ EXPORTS = (
     "dq_state_create", "dq_state_destroy", "dq_state_num_qubits", "dq_state_set", "dq_state_reset",
     "dq_apply_1q", "dq_apply_nq", "dq_prob_one", "dq_collapse", "dq_marginal_probs", "dq_inner",
     "dq_copy_to_host", "dq_synchronize", "dq_state_copy", "dq_apply_pauli_sum", "dq_pauli_inner",
-    "dq_compare_states", "dq_state_tensor",
+    "dq_compare_states", "dq_state_tensor", "dq_adjoint_sweep",
     "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
     "dq_get_fused_stats", "dq_set_fuse_min_cost", "dq_set_pair_fusion", "dq_top_k",
     "dq_batch_create", "dq_batch_destroy", "dq_batch_add_instance", "dq_batch_apply",
@@ -88,6 +88,7 @@ def lib() -> ctypes.CDLL:
         L.dq_state_copy.argtypes = [vp, vp]
         L.dq_compare_states.argtypes = [vp, vp, _dp, _dp, _dp, _dp]
         L.dq_state_tensor.argtypes = [vp, vp, _ip, vp, _ip]
+        L.dq_adjoint_sweep.argtypes = [vp, vp, ctypes.c_int, _ip, _ip, _dp, _ip, u64p, u64p, _dp, _dp]
         L.dq_apply_pauli_sum.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp]
         L.dq_pauli_inner.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp, _dp, _dp]
         L.dq_set_timing.argtypes = [vp, ctypes.c_int]

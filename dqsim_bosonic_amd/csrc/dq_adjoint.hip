@@ -1,0 +1,943 @@
+// dq_adjoint.hip -- the reverse sweep of the adjoint-gradient method as tile
+// passes over both states (no reference counterpart: the adjoint form of
+// the statevector path's gradient workload).
+//
+// For l = L-1 .. 0 the method needs g_l = Im <lam| G_l |psi> and then
+// psi <- U_l psi, lam <- U_l lam (U_l the inverse of gate l).  Every factor
+// acts on gate l's qubits only, so a run of steps whose qubits fit a set S of
+// b qubits can be applied tile by tile: a workgroup loads the 2^b-amplitude
+// tile of psi and of lam into LDS, runs the steps in order -- gate steps
+// rewrite both tiles, gradient steps sum conj(lam) (G psi) over the tile into
+// a per-tile partial -- and stores both tiles.  One pass reads and writes
+// each state once, instead of one flush sweep per state plus one inner-product
+// sweep per gradient.  Steps are taken out of order only across steps on
+// disjoint qubits, which commute (and leave <lam|G|psi> unchanged, G acting
+// on the step's qubits only).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dqsim_gpu.h"
+#include "dq_device.hpp"
+#include "dq_plan.hpp"
+#include "dq_state.hpp"
+
+using namespace dq;
+
+namespace {
+
+constexpr int kThreads = 256;       // reduction kernel
+constexpr int kPassThreads = 256;   // pass kernel workgroup
+constexpr int kMaxTileBits = 12;
+constexpr int kLowBits = 5;  // qubits 0..4 in every tile: loads in 512-byte runs (complex128)
+constexpr int kRegBits = 3;  // register-resident pass: amplitudes per thread and state = 2^kRegBits
+
+enum AdjKind : int {
+    A_GENERIC = 0,  // any gate (k <= 5): one output per amplitude through a barrier
+    A_GRAD = 1,     // gradient partial of the tile
+    A_DENSE1 = 2,   // 2x2 on bit[0], in place per pair
+    A_DENSE2 = 3,   // 4x4 on bit[0] (MSB), bit[1], in place per quad
+    A_DIAG = 4      // diagonal (k <= 2): elementwise phases
+};
+
+struct AdjStep {
+    int kind;      // AdjKind
+    int k;         // gate qubits
+    int bit[5];    // tile bits, bit[0] = MSB of the gate index
+    int mat;       // gate: offset (complex entries) into the matrix pool
+    int term0;     // gradient: first term
+    int nterms;    // gradient: term count
+    int slot;      // gradient: partial-sum slot within the pass
+};
+
+struct AdjTerm {
+    unsigned x, z;     // tile-local masks
+    double cre, cim;   // coefficient times i^{#Y}
+};
+
+struct AdjPass {
+    int b;           // tile bits
+    int n_outer;     // state qubits outside the tile
+    int tq[kMaxTileBits];  // tile bit -> qubit
+    int oq[64];            // outer bit -> qubit
+    int step0, nsteps;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cvt(cplx<double> u) {
+    return cplx<T>{T(u.re), T(u.im)};
+}
+
+template <typename T>
+__device__ __forceinline__ double block_sum2(double v, double *scratch) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kPassThreads / 64; ++w) s += scratch[w];
+    return s;
+}
+
+// One pass over every tile.  partials[slot * ntiles + tile] receives the
+// tile's Im sum of each gradient step.
+template <typename T, int B>
+__global__ __launch_bounds__(kPassThreads) void k_adjoint_pass(cplx<T> *__restrict__ psi, cplx<T> *__restrict__ lam,
+                                                           AdjPass pd, const AdjStep *__restrict__ steps,
+                                                           const cplx<double> *__restrict__ mats,
+                                                           const AdjTerm *__restrict__ terms,
+                                                           double *__restrict__ partials, uint64_t ntiles) {
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int kAmps = 1 << B;
+    constexpr int kPer = kAmps >= kPassThreads ? kAmps / kPassThreads : 1;  // amplitudes per thread (>= 1)
+    cplx<T> *P = reinterpret_cast<cplx<T> *>(smem_raw);
+    cplx<T> *Lm = P + kAmps;
+    double *scratch = reinterpret_cast<double *>(Lm + kAmps);
+    const int t = threadIdx.x;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t base = 0;
+        for (int o = 0; o < pd.n_outer; ++o) base |= ((tile >> o) & 1) << pd.oq[o];
+        auto gidx = [&](unsigned i) {
+            uint64_t g = base;
+#pragma unroll
+            for (int j = 0; j < B; ++j) g |= uint64_t((i >> j) & 1) << pd.tq[j];
+            return g;
+        };
+        __syncthreads();  // the previous tile's stores have read the LDS
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const unsigned i = unsigned(t + r * kPassThreads);
+            if constexpr (kAmps < kPassThreads) { if (i >= unsigned(kAmps)) break; }
+            const uint64_t g = gidx(i);
+            P[i] = psi[g];
+            Lm[i] = lam[g];
+        }
+        __syncthreads();
+        for (int s = 0; s < pd.nsteps; ++s) {
+            const AdjStep st = steps[pd.step0 + s];
+            if (st.kind == A_GRAD) {
+                const int nt = st.nterms;
+                double acc = 0.0;
+                if (nt <= 4) {
+                    // up to four terms held in registers (every generator here)
+                    AdjTerm t0{0, 0, 0.0, 0.0}, t1 = t0, t2 = t0, t3 = t0;
+                    if (nt > 0) t0 = terms[st.term0];
+                    if (nt > 1) t1 = terms[st.term0 + 1];
+                    if (nt > 2) t2 = terms[st.term0 + 2];
+                    if (nt > 3) t3 = terms[st.term0 + 3];
+#pragma unroll
+                    for (int r = 0; r < kPer; ++r) {
+                        const unsigned i = unsigned(t + r * kPassThreads);
+                        if constexpr (kAmps < kPassThreads) { if (i >= unsigned(kAmps)) break; }
+                        double gr = 0.0, gi = 0.0;
+                        auto term = [&](const AdjTerm &tq) {
+                            const unsigned y = i ^ tq.x;
+                            const cplx<T> v = P[y];
+                            const double sg = (__popc(tq.z & y) & 1) ? -1.0 : 1.0;
+                            const double vr = sg * double(v.re), vi = sg * double(v.im);
+                            gr += tq.cre * vr - tq.cim * vi;
+                            gi += tq.cre * vi + tq.cim * vr;
+                        };
+                        if (nt > 0) term(t0);
+                        if (nt > 1) term(t1);
+                        if (nt > 2) term(t2);
+                        if (nt > 3) term(t3);
+                        const double lr = double(Lm[i].re), li = double(Lm[i].im);
+                        acc += lr * gi - li * gr;  // Im(conj(lam) * g)
+                    }
+                } else {
+                    for (int r = 0; r < kPer; ++r) {
+                        const unsigned i = unsigned(t + r * kPassThreads);
+                        if constexpr (kAmps < kPassThreads) { if (i >= unsigned(kAmps)) break; }
+                        double gr = 0.0, gi = 0.0;
+                        for (int q = 0; q < nt; ++q) {
+                            const AdjTerm tq = terms[st.term0 + q];
+                            const unsigned y = i ^ tq.x;
+                            const cplx<T> v = P[y];
+                            const double sg = (__popc(tq.z & y) & 1) ? -1.0 : 1.0;
+                            const double vr = sg * double(v.re), vi = sg * double(v.im);
+                            gr += tq.cre * vr - tq.cim * vi;
+                            gi += tq.cre * vi + tq.cim * vr;
+                        }
+                        const double lr = double(Lm[i].re), li = double(Lm[i].im);
+                        acc += lr * gi - li * gr;
+                    }
+                }
+                const double v = block_sum2<T>(acc, scratch);
+                if (t == 0) partials[uint64_t(st.slot) * ntiles + tile] = v;
+                continue;
+            }
+            if (st.kind == A_DENSE1) {
+                const int pb = st.bit[0];
+                const cplx<T> u0 = cvt<T>(mats[st.mat]), u1 = cvt<T>(mats[st.mat + 1]);
+                const cplx<T> u2 = cvt<T>(mats[st.mat + 2]), u3 = cvt<T>(mats[st.mat + 3]);
+#pragma unroll
+                for (unsigned jj = unsigned(t); jj < unsigned(kAmps / 2); jj += kPassThreads) {
+                    const unsigned i0 = ((jj >> pb) << (pb + 1)) | (jj & ((1u << pb) - 1)), i1 = i0 | (1u << pb);
+                    const cplx<T> a0 = P[i0], a1 = P[i1], b0 = Lm[i0], b1 = Lm[i1];
+                    P[i0] = cadd(cmul(u0, a0), cmul(u1, a1));
+                    P[i1] = cadd(cmul(u2, a0), cmul(u3, a1));
+                    Lm[i0] = cadd(cmul(u0, b0), cmul(u1, b1));
+                    Lm[i1] = cadd(cmul(u2, b0), cmul(u3, b1));
+                }
+                __syncthreads();
+                continue;
+            }
+            if (st.kind == A_DENSE2) {
+                const int ph = st.bit[0], pl = st.bit[1];  // matrix MSB, LSB
+                const int lo = ph < pl ? ph : pl, hi = ph < pl ? pl : ph;
+                cplx<T> u[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) u[e] = cvt<T>(mats[st.mat + e]);
+#pragma unroll
+                for (unsigned jj = unsigned(t); jj < unsigned(kAmps / 4); jj += kPassThreads) {
+                    unsigned b0 = ((jj >> lo) << (lo + 1)) | (jj & ((1u << lo) - 1));
+                    b0 = ((b0 >> hi) << (hi + 1)) | (b0 & ((1u << hi) - 1));
+                    const unsigned idx[4] = {b0, b0 | (1u << pl), b0 | (1u << ph), b0 | (1u << ph) | (1u << pl)};
+                    cplx<T> a[4], c[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        a[e] = P[idx[e]];
+                        c[e] = Lm[idx[e]];
+                    }
+#pragma unroll
+                    for (int row = 0; row < 4; ++row) {
+                        cplx<T> x = cmul(u[4 * row], a[0]), y = cmul(u[4 * row], c[0]);
+#pragma unroll
+                        for (int col = 1; col < 4; ++col) {
+                            x = cadd(x, cmul(u[4 * row + col], a[col]));
+                            y = cadd(y, cmul(u[4 * row + col], c[col]));
+                        }
+                        P[idx[row]] = x;
+                        Lm[idx[row]] = y;
+                    }
+                }
+                __syncthreads();
+                continue;
+            }
+            if (st.kind == A_DIAG) {
+                const int k = st.k;
+                const int dim = 1 << k;
+                const cplx<T> d0 = cvt<T>(mats[st.mat]), d1 = cvt<T>(mats[st.mat + (dim + 1)]);
+                const cplx<T> d2 = k == 2 ? cvt<T>(mats[st.mat + 2 * (dim + 1)]) : d0;
+                const cplx<T> d3 = k == 2 ? cvt<T>(mats[st.mat + 3 * (dim + 1)]) : d0;
+                const int b0 = st.bit[0], b1 = k == 2 ? st.bit[1] : 0;
+#pragma unroll
+                for (int r = 0; r < kPer; ++r) {
+                    const unsigned i = unsigned(t + r * kPassThreads);
+                    if constexpr (kAmps < kPassThreads) { if (i >= unsigned(kAmps)) break; }
+                    const int row = k == 2 ? int(((i >> b0) & 1) << 1 | ((i >> b1) & 1)) : int((i >> b0) & 1);
+                    const cplx<T> ph = row == 0 ? d0 : row == 1 ? d1 : row == 2 ? d2 : d3;
+                    P[i] = cmul(ph, P[i]);
+                    Lm[i] = cmul(ph, Lm[i]);
+                }
+                __syncthreads();
+                continue;
+            }
+            // generic gate: each thread computes its kPer outputs of both
+            // tiles, then (after a barrier) writes them back
+            const int k = st.k, dim = 1 << k;
+            unsigned gm = 0;
+            for (int j2 = 0; j2 < k; ++j2) gm |= 1u << st.bit[j2];
+            cplx<T> op_[kPer], ol[kPer];
+#pragma unroll
+            for (int r = 0; r < kPer; ++r) {
+                const unsigned i = unsigned(t + r * kPassThreads);
+                if constexpr (kAmps < kPassThreads) { if (i >= unsigned(kAmps)) break; }
+                int row = 0;
+                for (int j2 = 0; j2 < k; ++j2) row = (row << 1) | int((i >> st.bit[j2]) & 1);
+                const unsigned b0 = i & ~gm;
+                cplx<T> ap{T(0), T(0)}, al{T(0), T(0)};
+                for (int c = 0; c < dim; ++c) {
+                    unsigned off = 0;
+                    for (int j2 = 0; j2 < k; ++j2)
+                        if ((c >> (k - 1 - j2)) & 1) off |= 1u << st.bit[j2];
+                    const cplx<T> uu = cvt<T>(mats[st.mat + row * dim + c]);
+                    ap = cadd(ap, cmul(uu, P[b0 | off]));
+                    al = cadd(al, cmul(uu, Lm[b0 | off]));
+                }
+                op_[r] = ap;
+                ol[r] = al;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kPer; ++r) {
+                const unsigned i = unsigned(t + r * kPassThreads);
+                if constexpr (kAmps < kPassThreads) { if (i >= unsigned(kAmps)) break; }
+                P[i] = op_[r];
+                Lm[i] = ol[r];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const unsigned i = unsigned(t + r * kPassThreads);
+            if constexpr (kAmps < kPassThreads) { if (i >= unsigned(kAmps)) break; }
+            const uint64_t g = gidx(i);
+            psi[g] = P[i];
+            lam[g] = Lm[i];
+        }
+    }
+}
+
+// ---- register-resident pass ----
+//
+// Each thread holds 2^RB amplitudes of each tile (the register bits of the
+// current layout, RB of the tile's B bits) and the 2^(B-RB) threads cover
+// the other bits.  Dense gates act on register bits in registers; diagonal
+// gates act on any bit (the phase follows from the amplitude's index);
+// gradient steps need their X bits among the register bits (the partner
+// amplitude is in the same thread) and reduce per wave with shuffles, so
+// they need no barrier.  A relayout step moves the tile through LDS to a new
+// set of register bits; it is the only step with barriers.
+
+enum AdjRKind : int { R_DENSE1 = 0, R_DENSE2 = 1, R_DIAG = 2, R_GRAD = 3, R_RELAYOUT = 4 };
+
+struct AdjRStep {
+    int kind;
+    int a, b;          // DENSE1: register slot a; DENSE2: slots a (matrix MSB), b
+    int k;             // DIAG: 1 or 2
+    int bits[2];       // DIAG: tile bits, MSB first
+    int mat;           // matrix offset (complex entries)
+    int term0, nterms, slot;  // GRAD
+    unsigned mask;     // RELAYOUT: register tile bits of the new layout
+};
+
+struct AdjRTerm {
+    unsigned xr;       // X bits as register-slot mask
+    unsigned xt, z;    // X bits and Z bits as tile-bit masks
+    unsigned pad;
+    double cre, cim;   // coefficient times i^{#Y}
+};
+
+__device__ __forceinline__ unsigned pdep_bits(unsigned v, unsigned mask) {
+    unsigned out = 0;
+    for (int b = 0; mask; ++b) {
+        const unsigned low = mask & (~mask + 1u);
+        if ((v >> b) & 1) out |= low;
+        mask &= mask - 1u;
+    }
+    return out;
+}
+
+template <typename T, int NR, int M>
+__device__ __forceinline__ void r_dense1(cplx<T> (&a)[NR], const cplx<T> u0, const cplx<T> u1, const cplx<T> u2,
+                                         const cplx<T> u3) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        if (j & (1 << M)) continue;
+        const int j1 = j | (1 << M);
+        const cplx<T> x0 = a[j], x1 = a[j1];
+        a[j] = cadd(cmul(u0, x0), cmul(u1, x1));
+        a[j1] = cadd(cmul(u2, x0), cmul(u3, x1));
+    }
+}
+
+template <typename T, int NR, int MH, int ML>
+__device__ __forceinline__ void r_dense2(cplx<T> (&a)[NR], const cplx<T> (&u)[16]) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        if (j & ((1 << MH) | (1 << ML))) continue;
+        const int idx[4] = {j, j | (1 << ML), j | (1 << MH), j | (1 << MH) | (1 << ML)};
+        cplx<T> x[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[e] = a[idx[e]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cplx<T> acc = cmul(u[4 * r], x[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(u[4 * r + c], x[c]));
+            a[idx[r]] = acc;
+        }
+    }
+}
+
+// sum over this thread's amplitudes of Im(conj(l) * (G p)) for one term set
+template <typename T, int NR, int XR>
+__device__ __forceinline__ double r_grad_term(const cplx<T> (&p)[NR], const cplx<T> (&l)[NR], const unsigned (&idx)[NR],
+                                              const AdjRTerm &tm) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const unsigned y = idx[j] ^ tm.xt;
+        const cplx<T> v = p[j ^ XR];
+        const double sg = (__popc(tm.z & y) & 1) ? -1.0 : 1.0;
+        const double vr = sg * double(v.re), vi = sg * double(v.im);
+        const double gr = tm.cre * vr - tm.cim * vi, gi = tm.cre * vi + tm.cim * vr;
+        acc += double(l[j].re) * gi - double(l[j].im) * gr;
+    }
+    return acc;
+}
+
+template <typename T, int NR>
+__device__ __forceinline__ double r_grad_any(const cplx<T> (&p)[NR], const cplx<T> (&l)[NR], const unsigned (&idx)[NR],
+                                             const AdjRTerm &tm) {
+    switch (tm.xr) {
+    case 0: return r_grad_term<T, NR, 0>(p, l, idx, tm);
+    case 1: return r_grad_term<T, NR, 1 % NR>(p, l, idx, tm);
+    case 2: return r_grad_term<T, NR, 2 % NR>(p, l, idx, tm);
+    case 3: return r_grad_term<T, NR, 3 % NR>(p, l, idx, tm);
+    case 4: return r_grad_term<T, NR, 4 % NR>(p, l, idx, tm);
+    case 5: return r_grad_term<T, NR, 5 % NR>(p, l, idx, tm);
+    case 6: return r_grad_term<T, NR, 6 % NR>(p, l, idx, tm);
+    default: return r_grad_term<T, NR, 7 % NR>(p, l, idx, tm);
+    }
+}
+
+template <typename T, int B, int RB>
+__global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restrict__ psi, cplx<T> *__restrict__ lam,
+                                                              AdjPass pd, const AdjRStep *__restrict__ steps,
+                                                              const cplx<double> *__restrict__ mats,
+                                                              const AdjRTerm *__restrict__ terms,
+                                                              double *__restrict__ partials, uint64_t ntiles,
+                                                              unsigned first_mask) {
+    constexpr int NT = 1 << (B - RB), NR = 1 << RB, W = NT / 64;
+    constexpr unsigned kTileMask = (1u << B) - 1u;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *LP = reinterpret_cast<cplx<T> *>(smem_raw);
+    cplx<T> *LL = LP + (1 << B);
+    const unsigned t = threadIdx.x;
+    const unsigned lane = t & 63, wave = t >> 6;
+    cplx<T> p[NR], l[NR];
+    unsigned idx[NR];
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t base = 0;
+        for (int o = 0; o < pd.n_outer; ++o) base |= ((tile >> o) & 1) << pd.oq[o];
+        auto gidx = [&](unsigned i) {
+            uint64_t g = base;
+#pragma unroll
+            for (int j = 0; j < B; ++j) g |= uint64_t((i >> j) & 1) << pd.tq[j];
+            return g;
+        };
+        auto layout = [&](unsigned mask) {
+            const unsigned tp = pdep_bits(t, kTileMask & ~mask);
+#pragma unroll
+            for (int j = 0; j < NR; ++j) idx[j] = tp | pdep_bits(unsigned(j), mask);
+        };
+        layout(first_mask);
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const uint64_t g = gidx(idx[j]);
+            p[j] = psi[g];
+            l[j] = lam[g];
+        }
+        for (int s = 0; s < pd.nsteps; ++s) {
+            const AdjRStep st = steps[pd.step0 + s];
+            switch (st.kind) {
+            case R_RELAYOUT: {
+                __syncthreads();  // every thread has read the previous layout's LDS data
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    LP[idx[j]] = p[j];
+                    LL[idx[j]] = l[j];
+                }
+                __syncthreads();
+                layout(st.mask);
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    p[j] = LP[idx[j]];
+                    l[j] = LL[idx[j]];
+                }
+                break;
+            }
+            case R_DENSE1: {
+                const cplx<T> u0 = cvt<T>(mats[st.mat]), u1 = cvt<T>(mats[st.mat + 1]);
+                const cplx<T> u2 = cvt<T>(mats[st.mat + 2]), u3 = cvt<T>(mats[st.mat + 3]);
+                switch (st.a) {
+                case 0: r_dense1<T, NR, 0>(p, u0, u1, u2, u3); r_dense1<T, NR, 0>(l, u0, u1, u2, u3); break;
+                case 1: r_dense1<T, NR, 1 % RB>(p, u0, u1, u2, u3); r_dense1<T, NR, 1 % RB>(l, u0, u1, u2, u3); break;
+                case 2: r_dense1<T, NR, 2 % RB>(p, u0, u1, u2, u3); r_dense1<T, NR, 2 % RB>(l, u0, u1, u2, u3); break;
+                default: r_dense1<T, NR, 3 % RB>(p, u0, u1, u2, u3); r_dense1<T, NR, 3 % RB>(l, u0, u1, u2, u3); break;
+                }
+                break;
+            }
+            case R_DENSE2: {
+                cplx<T> u[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) u[e] = cvt<T>(mats[st.mat + e]);
+                const int code = st.a * 4 + st.b;
+                switch (code) {
+                case 1: r_dense2<T, NR, 0, 1 % RB>(p, u); r_dense2<T, NR, 0, 1 % RB>(l, u); break;
+                case 2: r_dense2<T, NR, 0, 2 % RB>(p, u); r_dense2<T, NR, 0, 2 % RB>(l, u); break;
+                case 4: r_dense2<T, NR, 1 % RB, 0>(p, u); r_dense2<T, NR, 1 % RB, 0>(l, u); break;
+                case 6: r_dense2<T, NR, 1 % RB, 2 % RB>(p, u); r_dense2<T, NR, 1 % RB, 2 % RB>(l, u); break;
+                case 8: r_dense2<T, NR, 2 % RB, 0>(p, u); r_dense2<T, NR, 2 % RB, 0>(l, u); break;
+                default: r_dense2<T, NR, 2 % RB, 1 % RB>(p, u); r_dense2<T, NR, 2 % RB, 1 % RB>(l, u); break;
+                }
+                break;
+            }
+            case R_DIAG: {
+                const int dim = 1 << st.k;
+                const cplx<T> d0 = cvt<T>(mats[st.mat]), d1 = cvt<T>(mats[st.mat + (dim + 1)]);
+                const cplx<T> d2 = st.k == 2 ? cvt<T>(mats[st.mat + 2 * (dim + 1)]) : d0;
+                const cplx<T> d3 = st.k == 2 ? cvt<T>(mats[st.mat + 3 * (dim + 1)]) : d0;
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    const unsigned i = idx[j];
+                    const int row = st.k == 2 ? int(((i >> st.bits[0]) & 1) << 1 | ((i >> st.bits[1]) & 1))
+                                              : int((i >> st.bits[0]) & 1);
+                    const cplx<T> ph = row == 0 ? d0 : row == 1 ? d1 : row == 2 ? d2 : d3;
+                    p[j] = cmul(ph, p[j]);
+                    l[j] = cmul(ph, l[j]);
+                }
+                break;
+            }
+            default: {  // R_GRAD
+                double acc = 0.0;
+                for (int q = 0; q < st.nterms; ++q) acc += r_grad_any<T, NR>(p, l, idx, terms[st.term0 + q]);
+                for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+                if (lane == 0) partials[(uint64_t(st.slot) * ntiles + tile) * W + wave] = acc;
+                break;
+            }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const uint64_t g = gidx(idx[j]);
+            psi[g] = p[j];
+            lam[g] = l[j];
+        }
+        __syncthreads();  // the next tile's first relayout writes the LDS
+    }
+}
+
+// grads[slot_base + s] = sum over tiles of partials[s * ntiles + tile], in tile order.
+__global__ __launch_bounds__(kThreads) void k_adjoint_reduce(const double *__restrict__ partials, uint64_t ntiles,
+                                                             double *__restrict__ grads) {
+    __shared__ double scratch[kThreads / 64];
+    const int s = blockIdx.x;
+    double acc = 0.0;
+    for (uint64_t i = threadIdx.x; i < ntiles; i += kThreads) acc += partials[uint64_t(s) * ntiles + i];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) r += scratch[w];
+        grads[s] = r;
+    }
+}
+
+int afail(int code, const std::string &msg) { return dq::set_error(code, "adjoint sweep: " + msg); }
+
+#define ADJ_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess) return afail(DQ_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+template <typename T, int B>
+hipError_t launch_pass(unsigned grid, hipStream_t stream, void *psi, void *lam, const AdjPass &pd,
+                       const AdjStep *steps, const void *mats, const AdjTerm *terms, double *partials,
+                       uint64_t ntiles) {
+    const size_t shmem = 2 * sizeof(cplx<T>) * (size_t(1) << B) + 8 * (kPassThreads / 64);
+    if (shmem > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_adjoint_pass<T, B>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_adjoint_pass<T, B>), dim3(grid), dim3(kPassThreads), shmem, stream,
+                       static_cast<cplx<T> *>(psi), static_cast<cplx<T> *>(lam), pd, steps,
+                       static_cast<const cplx<double> *>(mats), terms, partials, ntiles);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_any(int b, unsigned grid, hipStream_t stream, void *psi, void *lam, const AdjPass &pd,
+                      const AdjStep *steps, const void *mats, const AdjTerm *terms, double *partials,
+                      uint64_t ntiles) {
+    switch (b) {
+    case 8: return launch_pass<T, 8>(grid, stream, psi, lam, pd, steps, mats, terms, partials, ntiles);
+    case 9: return launch_pass<T, 9>(grid, stream, psi, lam, pd, steps, mats, terms, partials, ntiles);
+    case 10: return launch_pass<T, 10>(grid, stream, psi, lam, pd, steps, mats, terms, partials, ntiles);
+    case 11: return launch_pass<T, 11>(grid, stream, psi, lam, pd, steps, mats, terms, partials, ntiles);
+    case 12: return launch_pass<T, 12>(grid, stream, psi, lam, pd, steps, mats, terms, partials, ntiles);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T, int B>
+hipError_t launch_reg_pass(unsigned grid, hipStream_t stream, void *psi, void *lam, const AdjPass &pd,
+                           const AdjRStep *steps, const void *mats, const AdjRTerm *terms, double *partials,
+                           uint64_t ntiles, unsigned first_mask) {
+    constexpr int RB = kRegBits;
+    const size_t shmem = 2 * sizeof(cplx<T>) * (size_t(1) << B);
+    if (shmem > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_adjoint_reg<T, B, RB>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_adjoint_reg<T, B, RB>), dim3(grid), dim3(1 << (B - RB)), shmem, stream,
+                       static_cast<cplx<T> *>(psi), static_cast<cplx<T> *>(lam), pd, steps,
+                       static_cast<const cplx<double> *>(mats), terms, partials, ntiles, first_mask);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_reg_any(int b, unsigned grid, hipStream_t stream, void *psi, void *lam, const AdjPass &pd,
+                          const AdjRStep *steps, const void *mats, const AdjRTerm *terms, double *partials,
+                          uint64_t ntiles, unsigned first_mask) {
+    switch (b) {
+    case 9: return launch_reg_pass<T, 9>(grid, stream, psi, lam, pd, steps, mats, terms, partials, ntiles, first_mask);
+    case 10: return launch_reg_pass<T, 10>(grid, stream, psi, lam, pd, steps, mats, terms, partials, ntiles, first_mask);
+    case 11: return launch_reg_pass<T, 11>(grid, stream, psi, lam, pd, steps, mats, terms, partials, ntiles, first_mask);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Register-resident form of one pass's steps: segments whose dense gates and
+// gradient X bits fit kRegBits register bits, joined by relayouts.  Returns
+// false when a step does not fit the form (gates on more than 2 qubits).
+bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm> &hterms, int b,
+                       std::vector<AdjRStep> &out, std::vector<AdjRTerm> &rterms, unsigned &first_mask) {
+    constexpr int RB = kRegBits;
+    const unsigned tile_mask = (1u << b) - 1u;
+    auto needs = [&](const AdjStep &x) -> unsigned {
+        switch (x.kind) {
+        case A_DENSE1: return 1u << x.bit[0];
+        case A_DENSE2: return (1u << x.bit[0]) | (1u << x.bit[1]);
+        case A_GRAD: {
+            unsigned m = 0;
+            for (int q = 0; q < x.nterms; ++q) m |= hterms[size_t(x.term0 + q)].x;
+            return m;
+        }
+        default: return 0;
+        }
+    };
+    for (int i = 0; i < nsteps; ++i) {
+        if (st[i].kind == A_GENERIC) return false;
+        if (__builtin_popcount(needs(st[i])) > RB) return false;
+    }
+    auto fill = [&](unsigned m) {
+        for (int bit = b - 1; bit >= 0 && __builtin_popcount(m) < RB; --bit)
+            if (!((m >> bit) & 1)) m |= 1u << bit;
+        return m & tile_mask;
+    };
+    // segment masks
+    std::vector<std::pair<int, unsigned>> segs;  // (first step, register mask)
+    unsigned cur = 0;
+    int start = 0;
+    for (int i = 0; i < nsteps; ++i) {
+        const unsigned q = needs(st[i]);
+        if (__builtin_popcount(cur | q) <= RB) {
+            cur |= q;
+            continue;
+        }
+        segs.emplace_back(start, fill(cur));
+        start = i;
+        cur = q;
+    }
+    segs.emplace_back(start, fill(cur));
+    first_mask = segs[0].second;
+    auto slot_of = [](unsigned mask, int bit) { return __builtin_popcount(mask & ((1u << bit) - 1u)); };
+    size_t si = 0;
+    unsigned mask = first_mask;
+    for (int i = 0; i < nsteps; ++i) {
+        if (si + 1 < segs.size() && segs[si + 1].first == i) {
+            ++si;
+            mask = segs[si].second;
+            AdjRStep r;
+            std::memset(&r, 0, sizeof r);
+            r.kind = R_RELAYOUT;
+            r.mask = mask;
+            out.push_back(r);
+        }
+        const AdjStep &x = st[i];
+        AdjRStep r;
+        std::memset(&r, 0, sizeof r);
+        r.mat = x.mat;
+        switch (x.kind) {
+        case A_DENSE1:
+            r.kind = R_DENSE1;
+            r.a = slot_of(mask, x.bit[0]);
+            break;
+        case A_DENSE2:
+            r.kind = R_DENSE2;
+            r.a = slot_of(mask, x.bit[0]);
+            r.b = slot_of(mask, x.bit[1]);
+            break;
+        case A_DIAG:
+            r.kind = R_DIAG;
+            r.k = x.k;
+            r.bits[0] = x.bit[0];
+            r.bits[1] = x.k == 2 ? x.bit[1] : 0;
+            break;
+        default: {  // A_GRAD
+            r.kind = R_GRAD;
+            r.term0 = int(rterms.size());
+            r.nterms = x.nterms;
+            r.slot = x.slot;
+            for (int q = 0; q < x.nterms; ++q) {
+                const AdjTerm &tm = hterms[size_t(x.term0 + q)];
+                AdjRTerm rt;
+                std::memset(&rt, 0, sizeof rt);
+                for (int bit = 0; bit < b; ++bit)
+                    if ((tm.x >> bit) & 1) rt.xr |= 1u << slot_of(mask, bit);
+                rt.xt = tm.x;
+                rt.z = tm.z;
+                rt.cre = tm.cre;
+                rt.cim = tm.cim;
+                rterms.push_back(rt);
+            }
+            break;
+        }
+        }
+        out.push_back(r);
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, const int *qubits, const double *mats,
+                     const int *n_terms, const uint64_t *xmask, const uint64_t *zmask, const double *coeff,
+                     double *grad) {
+    if (!psi || !lam || (n_ops > 0 && (!k || !qubits || !mats || !n_terms || !grad)))
+        return afail(DQ_ERR_ARG, "null argument");
+    if (psi == lam) return afail(DQ_ERR_ARG, "psi and lam must be distinct states");
+    if (psi->n != lam->n || psi->dtype != lam->dtype) return afail(DQ_ERR_ARG, "states differ in size or dtype");
+    const int n = psi->n;
+    if (n < 8) return afail(DQ_ERR_ARG, "the tiled adjoint sweep needs at least 8 qubits");
+    if (n_ops < 0) return afail(DQ_ERR_ARG, "n_ops must be >= 0");
+    // validate and lay out the ops
+    std::vector<size_t> mat_off(size_t(n_ops) + 1, 0), term_off(size_t(n_ops) + 1, 0);
+    std::vector<uint64_t> qmask(size_t(n_ops), 0);
+    for (int l = 0; l < n_ops; ++l) {
+        if (k[l] < 1 || k[l] > 5) return afail(DQ_ERR_ARG, "gate arity must be 1..5");
+        for (int j = 0; j < k[l]; ++j) {
+            const int q = qubits[5 * l + j];
+            if (q < 0 || q >= n) return afail(DQ_ERR_ARG, "qubit index out of range");
+            if ((qmask[size_t(l)] >> q) & 1) return afail(DQ_ERR_ARG, "gate qubits must be distinct");
+            qmask[size_t(l)] |= uint64_t(1) << q;
+        }
+        if (n_terms[l] < 0 || n_terms[l] > 64) return afail(DQ_ERR_ARG, "generator term count must be 0..64");
+        mat_off[size_t(l) + 1] = mat_off[size_t(l)] + (size_t(1) << (2 * k[l]));
+        term_off[size_t(l) + 1] = term_off[size_t(l)] + size_t(n_terms[l]);
+        for (size_t t = term_off[size_t(l)]; t < term_off[size_t(l) + 1]; ++t)
+            if ((xmask[t] | zmask[t]) & ~qmask[size_t(l)])
+                return afail(DQ_ERR_ARG, "a generator term acts outside its gate's qubits");
+    }
+    for (int l = 0; l < n_ops; ++l) grad[l] = 0.0;
+    if (n_ops == 0) return DQ_OK;
+    if (int rc = dq_synchronize(psi)) return rc;
+    if (int rc = dq_synchronize(lam)) return rc;
+    const int b = std::min(n, 11);
+    const uint64_t low = (uint64_t(1) << kLowBits) - 1;
+    // passes: ops in reverse order; an op joins the pass if its qubits fit
+    // the tile and miss every qubit of an op left for a later pass
+    std::vector<std::vector<int>> passes;
+    std::vector<uint64_t> pass_S;
+    {
+        std::vector<int> remaining;
+        for (int l = n_ops - 1; l >= 0; --l) remaining.push_back(l);
+        while (!remaining.empty()) {
+            uint64_t S = low, blocked = 0;
+            std::vector<int> take, rest;
+            for (int l : remaining) {
+                const uint64_t q = qmask[size_t(l)];
+                if (!(q & blocked) && __builtin_popcountll(S | q) <= b) {
+                    S |= q;
+                    take.push_back(l);
+                } else {
+                    blocked |= q;
+                    rest.push_back(l);
+                }
+            }
+            if (take.empty()) return afail(DQ_ERR_ARG, "a gate does not fit the tile");
+            // fill the tile with the lowest unused qubits
+            for (int q = 0; q < n && __builtin_popcountll(S) < b; ++q) S |= uint64_t(1) << q;
+            passes.push_back(std::move(take));
+            pass_S.push_back(S);
+            remaining.swap(rest);
+        }
+    }
+    // device buffers: matrices (complex128), steps, terms, partials, grads
+    std::vector<cplx<double>> hmats(mat_off[size_t(n_ops)]);
+    std::memcpy(hmats.data(), mats, hmats.size() * sizeof(cplx<double>));
+    const uint64_t ntiles = uint64_t(1) << (n - b);
+    size_t max_slots = 1;
+    std::vector<AdjStep> hsteps;
+    std::vector<AdjTerm> hterms;
+    std::vector<AdjPass> hpasses;
+    std::vector<AdjRStep> rsteps;  // register-resident form of the passes that allow it
+    std::vector<AdjRTerm> rterms;
+    std::vector<AdjPass> rpasses;
+    std::vector<unsigned> first_masks;
+    std::vector<bool> use_reg;
+    std::vector<std::vector<int>> pass_grad_ops;
+    for (size_t p = 0; p < passes.size(); ++p) {
+        AdjPass pd;
+        std::memset(&pd, 0, sizeof pd);
+        pd.b = b;
+        int pos[64];
+        int tb = 0, ob = 0;
+        for (int q = 0; q < n; ++q) {
+            if ((pass_S[p] >> q) & 1) {
+                pd.tq[tb] = q;
+                pos[q] = tb++;
+            } else {
+                pd.oq[ob++] = q;
+                pos[q] = -1;
+            }
+        }
+        pd.n_outer = ob;
+        pd.step0 = int(hsteps.size());
+        std::vector<int> gops;
+        for (int l : passes[p]) {
+            if (n_terms[l] > 0) {
+                AdjStep g;
+                std::memset(&g, 0, sizeof g);
+                g.kind = A_GRAD;
+                g.term0 = int(hterms.size());
+                g.nterms = n_terms[l];
+                g.slot = int(gops.size());
+                for (size_t t = term_off[size_t(l)]; t < term_off[size_t(l) + 1]; ++t) {
+                    AdjTerm tm;
+                    tm.x = tm.z = 0;
+                    for (int q = 0; q < n; ++q) {
+                        if ((xmask[t] >> q) & 1) tm.x |= 1u << pos[q];
+                        if ((zmask[t] >> q) & 1) tm.z |= 1u << pos[q];
+                    }
+                    const int ny = __builtin_popcountll(xmask[t] & zmask[t]) & 3;
+                    double re = coeff[2 * t], im = coeff[2 * t + 1];
+                    for (int j = 0; j < ny; ++j) {  // multiply by i
+                        const double tmp = re;
+                        re = -im;
+                        im = tmp;
+                    }
+                    tm.cre = re;
+                    tm.cim = im;
+                    hterms.push_back(tm);
+                }
+                hsteps.push_back(g);
+                gops.push_back(l);
+            }
+            AdjStep s;
+            std::memset(&s, 0, sizeof s);
+            s.k = k[l];
+            for (int j = 0; j < k[l]; ++j) s.bit[j] = pos[qubits[5 * l + j]];
+            s.mat = int(mat_off[size_t(l)]);
+            bool diag = true;
+            const int dim = 1 << k[l];
+            for (int r = 0; r < dim && diag; ++r)
+                for (int c = 0; c < dim; ++c)
+                    if (r != c && (hmats[mat_off[size_t(l)] + size_t(r * dim + c)].re != 0.0 ||
+                                   hmats[mat_off[size_t(l)] + size_t(r * dim + c)].im != 0.0)) {
+                        diag = false;
+                        break;
+                    }
+            s.kind = (diag && k[l] <= 2) ? A_DIAG : k[l] == 1 ? A_DENSE1 : k[l] == 2 ? A_DENSE2 : A_GENERIC;
+            hsteps.push_back(s);
+        }
+        pd.nsteps = int(hsteps.size()) - pd.step0;
+        max_slots = std::max(max_slots, gops.size());
+        AdjPass rp = pd;
+        rp.step0 = int(rsteps.size());
+        unsigned fm = 0;
+        const bool reg = b >= 9 && to_register_steps(hsteps.data() + pd.step0, pd.nsteps, hterms, b, rsteps, rterms, fm);
+        if (!reg) rsteps.resize(size_t(rp.step0));
+        rp.nsteps = int(rsteps.size()) - rp.step0;
+        use_reg.push_back(reg);
+        rpasses.push_back(rp);
+        first_masks.push_back(fm);
+        hpasses.push_back(pd);
+        pass_grad_ops.push_back(std::move(gops));
+    }
+    void *d_mats = nullptr, *d_steps = nullptr, *d_terms = nullptr, *d_rsteps = nullptr, *d_rterms = nullptr;
+    double *d_part = nullptr, *d_grads = nullptr;
+    auto release = [&]() {
+        if (d_rsteps) (void)hipFree(d_rsteps);
+        if (d_rterms) (void)hipFree(d_rterms);
+        if (d_mats) (void)hipFree(d_mats);
+        if (d_steps) (void)hipFree(d_steps);
+        if (d_terms) (void)hipFree(d_terms);
+        if (d_part) (void)hipFree(d_part);
+        if (d_grads) (void)hipFree(d_grads);
+    };
+    const size_t slots_total = std::max<size_t>(1, [&] {
+        size_t s = 0;
+        for (const auto &g : pass_grad_ops) s += g.size();
+        return s;
+    }());
+    if (hipMalloc(&d_mats, std::max<size_t>(1, hmats.size()) * sizeof(cplx<double>)) != hipSuccess ||
+        hipMalloc(&d_steps, std::max<size_t>(1, hsteps.size()) * sizeof(AdjStep)) != hipSuccess ||
+        hipMalloc(&d_terms, std::max<size_t>(1, hterms.size()) * sizeof(AdjTerm)) != hipSuccess ||
+        hipMalloc(&d_rsteps, std::max<size_t>(1, rsteps.size()) * sizeof(AdjRStep)) != hipSuccess ||
+        hipMalloc(&d_rterms, std::max<size_t>(1, rterms.size()) * sizeof(AdjRTerm)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&d_part), max_slots * ntiles * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void **>(&d_grads), slots_total * sizeof(double)) != hipSuccess) {
+        release();
+        return afail(DQ_ERR_OOM, "hipMalloc of the sweep's buffers failed");
+    }
+    hipStream_t stream = psi->stream;
+    int rc = DQ_OK;
+    auto run = [&]() -> int {
+        ADJ_TRY(hipMemcpyAsync(d_mats, hmats.data(), hmats.size() * sizeof(cplx<double>), hipMemcpyHostToDevice,
+                               stream));
+        ADJ_TRY(hipMemcpyAsync(d_steps, hsteps.data(), hsteps.size() * sizeof(AdjStep), hipMemcpyHostToDevice,
+                               stream));
+        if (!hterms.empty())
+            ADJ_TRY(hipMemcpyAsync(d_terms, hterms.data(), hterms.size() * sizeof(AdjTerm), hipMemcpyHostToDevice,
+                                   stream));
+        if (!rsteps.empty())
+            ADJ_TRY(hipMemcpyAsync(d_rsteps, rsteps.data(), rsteps.size() * sizeof(AdjRStep), hipMemcpyHostToDevice,
+                                   stream));
+        if (!rterms.empty())
+            ADJ_TRY(hipMemcpyAsync(d_rterms, rterms.data(), rterms.size() * sizeof(AdjRTerm), hipMemcpyHostToDevice,
+                                   stream));
+        const unsigned grid = unsigned(std::min<uint64_t>(ntiles, 8192));
+        size_t slot_base = 0;
+        for (size_t p = 0; p < hpasses.size(); ++p) {
+            hipError_t e;
+            uint64_t count = ntiles;  // partials per gradient slot
+            if (use_reg[p]) {
+                count = ntiles * uint64_t((1 << (b - kRegBits)) / 64);
+                e = psi->dtype == DQ_C128
+                        ? launch_reg_any<double>(b, grid, stream, psi->amps, lam->amps, rpasses[p],
+                                                 static_cast<const AdjRStep *>(d_rsteps), d_mats,
+                                                 static_cast<const AdjRTerm *>(d_rterms), d_part, ntiles, first_masks[p])
+                        : launch_reg_any<float>(b, grid, stream, psi->amps, lam->amps, rpasses[p],
+                                                static_cast<const AdjRStep *>(d_rsteps), d_mats,
+                                                static_cast<const AdjRTerm *>(d_rterms), d_part, ntiles, first_masks[p]);
+            } else {
+                e = psi->dtype == DQ_C128
+                        ? launch_any<double>(b, grid, stream, psi->amps, lam->amps, hpasses[p],
+                                             static_cast<const AdjStep *>(d_steps), d_mats,
+                                             static_cast<const AdjTerm *>(d_terms), d_part, ntiles)
+                        : launch_any<float>(b, grid, stream, psi->amps, lam->amps, hpasses[p],
+                                            static_cast<const AdjStep *>(d_steps), d_mats,
+                                            static_cast<const AdjTerm *>(d_terms), d_part, ntiles);
+            }
+            ADJ_TRY(e);
+            const size_t ns = pass_grad_ops[p].size();
+            if (ns) {
+                hipLaunchKernelGGL(k_adjoint_reduce, dim3(unsigned(ns)), dim3(kThreads), 0, stream, d_part, count,
+                                   d_grads + slot_base);
+                ADJ_TRY(hipGetLastError());
+            }
+            slot_base += ns;
+            psi->launches++;
+            psi->sweeps++;
+        }
+        std::vector<double> hg(slots_total);
+        ADJ_TRY(hipMemcpyAsync(hg.data(), d_grads, slots_total * sizeof(double), hipMemcpyDeviceToHost, stream));
+        ADJ_TRY(hipStreamSynchronize(stream));
+        size_t s = 0;
+        for (const auto &g : pass_grad_ops)
+            for (int l : g) grad[l] = hg[s++];
+        return DQ_OK;
+    };
+    rc = run();
+    (void)hipStreamSynchronize(stream);
+    release();
+    return rc;
+}
+
+}  // extern "C"

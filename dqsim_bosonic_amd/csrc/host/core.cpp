@@ -1845,6 +1845,10 @@ class Adjoint {
 
     int num_params() const { return n_params_; }
 
+    // true (default): the reverse sweep runs as tile passes over both states
+    // (dq_adjoint_sweep, n >= 8); false: one inner product per parameter.
+    void set_tiled_sweep(bool on) { sweep_ = on; }
+
     double expectation(const std::vector<double> &params) {
         forward(params);
         return energy();
@@ -1855,6 +1859,33 @@ class Adjoint {
         check(dq_apply_pauli_sum(lam_.s, psi_.s, h_.size(), h_.x.data(), h_.z.data(), h_.c.data()), "H psi");
         const double e = energy_with_lambda();
         std::vector<double> grad(size_t(n_params_), 0.0);
+        if (n_ >= 8 && sweep_) {
+            // the whole reverse sweep as tile passes over both states
+            const size_t L = ops_.size();
+            std::vector<int> ks(L), qs(5 * L, 0), nt(L, 0);
+            std::vector<double> mats, coeff, g(L, 0.0);
+            std::vector<uint64_t> xm, zm;
+            for (size_t l = 0; l < L; ++l) {
+                const POp &op = ops_[l];
+                const Mat ud = dagger(matrix(op, params));
+                ks[l] = ud.k;
+                for (int j = 0; j < ud.k; ++j) qs[5 * l + size_t(j)] = op.qubits[size_t(j)];
+                mats.insert(mats.end(), ud.v.begin(), ud.v.end());
+                if (op.param >= 0) {
+                    const PauliSumH gen = generator(op);
+                    nt[l] = gen.size();
+                    xm.insert(xm.end(), gen.x.begin(), gen.x.end());
+                    zm.insert(zm.end(), gen.z.begin(), gen.z.end());
+                    coeff.insert(coeff.end(), gen.c.begin(), gen.c.end());
+                }
+            }
+            check(dq_adjoint_sweep(psi_.s, lam_.s, int(L), ks.data(), qs.data(), mats.data(), nt.data(), xm.data(),
+                                   zm.data(), coeff.data(), g.data()),
+                  "adjoint sweep");
+            for (size_t l = 0; l < L; ++l)
+                if (ops_[l].param >= 0) grad[size_t(ops_[l].param)] += g[l];
+            return py::make_tuple(e, grad);
+        }
         for (size_t l = ops_.size(); l-- > 0;) {
             const POp &op = ops_[l];
             if (op.param >= 0) {
@@ -1882,6 +1913,7 @@ class Adjoint {
     std::vector<POp> ops_;
     PauliSumH h_;
     int n_params_ = 0;
+    bool sweep_ = true;
 
     std::shared_ptr<DeviceState> psi_share() {
         // a fresh state holding a copy of psi, owned by the result
@@ -2109,6 +2141,7 @@ PYBIND11_MODULE(_core, m) {
         .def("set_circuit", &Adjoint::set_circuit)
         .def("set_observable", &Adjoint::set_observable)
         .def("num_params", &Adjoint::num_params)
+        .def("set_tiled_sweep", &Adjoint::set_tiled_sweep)
         .def("expectation", &Adjoint::expectation)
         .def("gradient", &Adjoint::gradient)
         .def("state", &Adjoint::state);

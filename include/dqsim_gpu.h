@@ -114,6 +114,19 @@ int dq_apply_pauli_sum(dq_state *dst, dq_state *src, int n_terms, const uint64_t
 int dq_pauli_inner(dq_state *bra, dq_state *ket, int n_terms, const uint64_t *xmask,
                    const uint64_t *zmask, const double *coeff, double *re, double *im);
 
+/* Reverse sweep of the adjoint-gradient method (no reference counterpart):
+ * for l = n_ops-1 down to 0, grad[l] = Im <lam| G_l |psi> when n_terms[l] > 0
+ * (else 0), then psi <- U_l psi and lam <- U_l lam.  Op l acts on
+ * qubits[5*l .. 5*l+k[l]) (first = MSB) with the 2^k x 2^k matrix U_l taken
+ * from `mats` (row-major complex128, ops concatenated) -- normally the
+ * inverse of forward gate l; G_l = sum of n_terms[l] Pauli terms (xmask,
+ * zmask, (re, im) coeff, as dq_pauli_inner takes them, ops concatenated)
+ * acting on op l's qubits only.  Runs as tile passes that read and write each
+ * state once per pass; n >= 8. */
+int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, const int *qubits,
+                     const double *mats, const int *n_terms, const uint64_t *xmask, const uint64_t *zmask,
+                     const double *coeff, double *grad);
+
 /* Lazy results (no reference counterpart): the k (1..2^20) most probable
  * basis states, p = |amp|^2 as probabilities() computes it, sorted by
  * decreasing p, ties by increasing index; the distribution is never copied

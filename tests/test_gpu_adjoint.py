@@ -135,3 +135,62 @@ def test_torch_autograd_function():
     _, g = est.gradient(th)
     assert abs(float(e) - est.expectation(th)) < 1e-12
     assert np.max(np.abs(t.grad.numpy() - g)) < 1e-14
+
+
+def wide_circuit(n, seed, layers=3):
+    """Every trainable gate kind on qubits spread over the whole register, so
+    that reverse tile passes hold gates on outer and inner qubits."""
+    rng = np.random.default_rng(seed)
+    pc = A.ParametricCircuit(n)
+    k = 0
+    for layer in range(layers):
+        for q in range(n):
+            pc.ry(q, param=k); k += 1
+            pc.rz(q, param=k); k += 1
+        for q in range(layer % 2, n - 1, 2):
+            pc.rzz(q, q + 1, param=k); k += 1
+        a, b, c, d = (int(x) for x in rng.choice(n, 4, replace=False))
+        pc.crx(a, b, param=k); k += 1
+        pc.cry(c, d, param=k); k += 1
+        pc.crz(b, c, param=k); k += 1
+        pc.cp(d, a, param=k); k += 1
+        pc.rxx(a, d, param=k); k += 1
+        pc.p(c, param=k); k += 1
+        pc.rx(b, param=k); k += 1
+        pc.cx(n - 1, 0)
+        pc.h(n // 2)
+    H = A.PauliSum.zz_chain(n) + A.PauliSum.x_sum(n, 0.7) + \
+        A.PauliSum.pauli({0: "Y", n // 2: "X", n - 1: "Z"}, 0.3)
+    return pc, H, rng.uniform(0, 2 * math.pi, pc.num_params)
+
+
+@pytest.mark.parametrize("n", [8, 11, 12, 13, 15])
+def test_tiled_reverse_sweep_equals_per_parameter_sweep(n):
+    pc, H, th = wide_circuit(n, n)
+    est = A.Estimator(pc, H)
+    e_t, g_t = est.gradient(th)
+    est._eng.set_tiled_sweep(False)
+    e_p, g_p = est.gradient(th)
+    assert abs(e_t - e_p) <= 1e-13
+    assert np.max(np.abs(g_t - g_p)) <= 1e-12
+    # the sweep undoes every gate: psi ends at |0...0>
+    est._eng.set_tiled_sweep(True)
+    est.gradient(th)
+    psi = est._eng.state().statevector
+    assert abs(abs(psi[0]) - 1.0) <= 1e-12
+
+
+def test_tiled_reverse_sweep_against_parameter_shift():
+    pc, H, th = wide_circuit(10, 3, layers=2)
+    est = A.Estimator(pc, H)
+    _, g = est.gradient(th)
+    idx = list(range(0, pc.num_params, 7))
+    ps = est.parameter_shift(th, idx)
+    assert np.max(np.abs(g[idx] - ps)) <= 1e-10
+
+
+def test_tiled_reverse_sweep_complex64():
+    pc, H, th = wide_circuit(12, 5)
+    _, g64 = A.Estimator(pc, H, dtype="c128").gradient(th)
+    _, g32 = A.Estimator(pc, H, dtype="c64").gradient(th)
+    assert np.max(np.abs(g32 - g64)) <= 1e-4 * max(1.0, np.max(np.abs(g64)))
