@@ -50,7 +50,7 @@ def core_path() -> str:
 ENGINE_SOURCES = (("dq_engine.hip", "off"), ("dq_tile_f64.hip", "off"), ("dq_tile_f64_fma.hip", "fast"),
                   ("dq_tile_f32.hip", "fast"), ("dq_tile_f32_mfma.hip", "fast"), ("dq_batch.hip", "off"),
                   ("dq_shots.hip", "off"), ("dq_mps.hip", "fast"),
-                  ("dq_adjoint.hip", "off"))
+                  ("dq_adjoint.hip", "fast"))
 
 
 def build_engine() -> str:

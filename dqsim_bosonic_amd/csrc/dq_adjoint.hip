@@ -12,7 +12,8 @@
 // each state once, instead of one flush sweep per state plus one inner-product
 // sweep per gradient.  Steps are taken out of order only across steps on
 // disjoint qubits, which commute (and leave <lam|G|psi> unchanged, G acting
-// on the step's qubits only).
+// on the step's qubits only).  Compiled with contraction (FMA): the gradients
+// agree with the per-parameter inner products to rounding.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -296,16 +297,16 @@ __global__ __launch_bounds__(kPassThreads) void k_adjoint_pass(cplx<T> *__restri
 // they need no barrier.  A relayout step moves the tile through LDS to a new
 // set of register bits; it is the only step with barriers.
 
-enum AdjRKind : int { R_DENSE1 = 0, R_DENSE2 = 1, R_DIAG = 2, R_GRAD = 3, R_RELAYOUT = 4 };
+enum AdjRKind : int { R_DENSE1 = 0, R_DENSE2 = 1, R_DIAG = 2, R_GRAD = 3, R_RELAYOUT = 4, R_DTABLE = 5 };
 
 struct AdjRStep {
     int kind;
-    int a, b;          // DENSE1: register slot a; DENSE2: slots a (matrix MSB), b
+    int a, b;          // DENSE1: register slot a; DENSE2: slots a (matrix MSB), b; GRAD: a = 1 if all terms are Z strings
     int k;             // DIAG: 1 or 2
     int bits[2];       // DIAG: tile bits, MSB first
     int mat;           // matrix offset (complex entries)
     int term0, nterms, slot;  // GRAD
-    unsigned mask;     // RELAYOUT: register tile bits of the new layout
+    unsigned mask;     // RELAYOUT: register tile bits of the new layout; DTABLE: the table's tile bits
 };
 
 struct AdjRTerm {
@@ -405,6 +406,7 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
     const unsigned lane = t & 63, wave = t >> 6;
     cplx<T> p[NR], l[NR];
     unsigned idx[NR];
+    unsigned tpart = 0, cur_mask = 0;  // this thread's tile bits and the register bits of the layout
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         uint64_t base = 0;
         for (int o = 0; o < pd.n_outer; ++o) base |= ((tile >> o) & 1) << pd.oq[o];
@@ -415,9 +417,10 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
             return g;
         };
         auto layout = [&](unsigned mask) {
-            const unsigned tp = pdep_bits(t, kTileMask & ~mask);
+            tpart = pdep_bits(t, kTileMask & ~mask);
+            cur_mask = mask;
 #pragma unroll
-            for (int j = 0; j < NR; ++j) idx[j] = tp | pdep_bits(unsigned(j), mask);
+            for (int j = 0; j < NR; ++j) idx[j] = tpart | pdep_bits(unsigned(j), mask);
         };
         layout(first_mask);
 #pragma unroll
@@ -471,6 +474,41 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
                 }
                 break;
             }
+            case R_DTABLE: {
+                // a product of diagonal gates as one phase table over the tile bits in st.mask
+                const unsigned m = st.mask;
+                unsigned pt = 0;
+                {
+                    unsigned mm = m, pos = 0;
+                    while (mm) {
+                        const unsigned low = mm & (~mm + 1u);
+                        if (tpart & low) pt |= 1u << pos;
+                        ++pos;
+                        mm &= mm - 1u;
+                    }
+                }
+                unsigned c[RB];
+                {
+                    unsigned cm = cur_mask;
+#pragma unroll
+                    for (int r = 0; r < RB; ++r) {
+                        const unsigned low = cm & (~cm + 1u);
+                        c[r] = (m & low) ? (1u << __popc(m & (low - 1u))) : 0u;
+                        cm &= cm - 1u;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    unsigned row = pt;
+#pragma unroll
+                    for (int r = 0; r < RB; ++r)
+                        if ((j >> r) & 1) row |= c[r];
+                    const cplx<T> ph = cvt<T>(mats[st.mat + int(row)]);
+                    p[j] = cmul(ph, p[j]);
+                    l[j] = cmul(ph, l[j]);
+                }
+                break;
+            }
             case R_DIAG: {
                 const int dim = 1 << st.k;
                 const cplx<T> d0 = cvt<T>(mats[st.mat]), d1 = cvt<T>(mats[st.mat + (dim + 1)]);
@@ -489,7 +527,27 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
             }
             default: {  // R_GRAD
                 double acc = 0.0;
-                for (int q = 0; q < st.nterms; ++q) acc += r_grad_any<T, NR>(p, l, idx, terms[st.term0 + q]);
+                if (st.a) {
+                    // every term diagonal (Z strings): Im(conj(l) c s p) = s (cre Im(conj(l) p) + cim Re(conj(l) p))
+                    double wr[NR], wi[NR];
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) {
+                        const double pr = double(p[j].re), pi = double(p[j].im);
+                        const double lr = double(l[j].re), li = double(l[j].im);
+                        wr[j] = lr * pr + li * pi;
+                        wi[j] = lr * pi - li * pr;
+                    }
+                    for (int q = 0; q < st.nterms; ++q) {
+                        const AdjRTerm tm = terms[st.term0 + q];
+#pragma unroll
+                        for (int j = 0; j < NR; ++j) {
+                            const double v = tm.cre * wi[j] + tm.cim * wr[j];
+                            acc += (__popc(tm.z & idx[j]) & 1) ? -v : v;
+                        }
+                    }
+                } else {
+                    for (int q = 0; q < st.nterms; ++q) acc += r_grad_any<T, NR>(p, l, idx, terms[st.term0 + q]);
+                }
                 for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
                 if (lane == 0) partials[(uint64_t(st.slot) * ntiles + tile) * W + wave] = acc;
                 break;
@@ -594,7 +652,9 @@ hipError_t launch_reg_any(int b, unsigned grid, hipStream_t stream, void *psi, v
 // gradient X bits fit kRegBits register bits, joined by relayouts.  Returns
 // false when a step does not fit the form (gates on more than 2 qubits).
 bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm> &hterms, int b,
-                       std::vector<AdjRStep> &out, std::vector<AdjRTerm> &rterms, unsigned &first_mask) {
+                       std::vector<cplx<double>> &hmats, std::vector<AdjRStep> &out, std::vector<AdjRTerm> &rterms,
+                       unsigned &first_mask) {
+    const size_t out0 = out.size();
     constexpr int RB = kRegBits;
     const unsigned tile_mask = (1u << b) - 1u;
     auto needs = [&](const AdjStep &x) -> unsigned {
@@ -672,6 +732,9 @@ bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm>
             r.term0 = int(rterms.size());
             r.nterms = x.nterms;
             r.slot = x.slot;
+            r.a = 1;  // 1: every term is a Z string
+            for (int q = 0; q < x.nterms; ++q)
+                if (hterms[size_t(x.term0 + q)].x) r.a = 0;
             for (int q = 0; q < x.nterms; ++q) {
                 const AdjTerm &tm = hterms[size_t(x.term0 + q)];
                 AdjRTerm rt;
@@ -689,6 +752,60 @@ bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm>
         }
         out.push_back(r);
     }
+    // Runs of diagonal gates and Z-string gradients: every such gradient is
+    // unchanged by diagonal gates (they commute with Z strings and act on
+    // both states), so the run becomes its gradients followed by the product
+    // of its gates as one phase table.
+    std::vector<AdjRStep> res(out.begin(), out.begin() + long(out0));
+    size_t i = out0;
+    while (i < out.size()) {
+        const auto in_run = [&](const AdjRStep &x) { return x.kind == R_DIAG || (x.kind == R_GRAD && x.a == 1); };
+        if (!in_run(out[i])) {
+            res.push_back(out[i++]);
+            continue;
+        }
+        size_t j = i;
+        std::vector<AdjRStep> grads, diags;
+        while (j < out.size() && in_run(out[j])) (out[j].kind == R_DIAG ? diags : grads).push_back(out[j++]);
+        for (const AdjRStep &g : grads) res.push_back(g);
+        if (diags.size() == 1) {
+            res.push_back(diags[0]);
+        } else if (diags.size() > 1) {
+            unsigned m = 0;
+            for (const AdjRStep &d : diags) {
+                m |= 1u << d.bits[0];
+                if (d.k == 2) m |= 1u << d.bits[1];
+            }
+            const int nb = __builtin_popcount(m);
+            std::vector<int> tbits;
+            for (int bit = 0; bit < b; ++bit)
+                if ((m >> bit) & 1) tbits.push_back(bit);
+            AdjRStep tstep;
+            std::memset(&tstep, 0, sizeof tstep);
+            tstep.kind = R_DTABLE;
+            tstep.mask = m;
+            tstep.mat = int(hmats.size());
+            for (unsigned row = 0; row < (1u << nb); ++row) {
+                unsigned tile = 0;  // the row's tile-bit values
+                for (int e = 0; e < nb; ++e)
+                    if ((row >> e) & 1) tile |= 1u << tbits[size_t(e)];
+                double re = 1.0, im = 0.0;
+                for (const AdjRStep &d : diags) {
+                    const int dim = 1 << d.k;
+                    const int dr = d.k == 2 ? int(((tile >> d.bits[0]) & 1) << 1 | ((tile >> d.bits[1]) & 1))
+                                            : int((tile >> d.bits[0]) & 1);
+                    const cplx<double> ph = hmats[size_t(d.mat + dr * (dim + 1))];
+                    const double nr = ph.re * re - ph.im * im, ni = ph.re * im + ph.im * re;
+                    re = nr;
+                    im = ni;
+                }
+                hmats.push_back(cplx<double>{re, im});
+            }
+            res.push_back(tstep);
+        }
+        i = j;
+    }
+    out.swap(res);
     return true;
 }
 
@@ -841,7 +958,8 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
         AdjPass rp = pd;
         rp.step0 = int(rsteps.size());
         unsigned fm = 0;
-        const bool reg = b >= 9 && to_register_steps(hsteps.data() + pd.step0, pd.nsteps, hterms, b, rsteps, rterms, fm);
+        const bool reg =
+            b >= 9 && to_register_steps(hsteps.data() + pd.step0, pd.nsteps, hterms, b, hmats, rsteps, rterms, fm);
         if (!reg) rsteps.resize(size_t(rp.step0));
         rp.nsteps = int(rsteps.size()) - rp.step0;
         use_reg.push_back(reg);
