@@ -391,7 +391,7 @@ __device__ __forceinline__ double r_grad_any(const cplx<T> (&p)[NR], const cplx<
 }
 
 template <typename T, int B, int RB>
-__global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restrict__ psi, cplx<T> *__restrict__ lam,
+__global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__restrict__ psi, cplx<T> *__restrict__ lam,
                                                               AdjPass pd, const AdjRStep *__restrict__ steps,
                                                               const cplx<double> *__restrict__ mats,
                                                               const AdjRTerm *__restrict__ terms,
@@ -405,27 +405,46 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
     const unsigned t = threadIdx.x;
     const unsigned lane = t & 63, wave = t >> 6;
     cplx<T> p[NR], l[NR];
-    unsigned idx[NR];
     unsigned tpart = 0, cur_mask = 0;  // this thread's tile bits and the register bits of the layout
+    // tile index of amplitude j: the thread's bits | j's register bits (the
+    // layout is the same for the whole workgroup, so the latter is scalar)
+    auto ix = [&](int j) {
+        unsigned cm = __builtin_amdgcn_readfirstlane(cur_mask), o = 0;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const unsigned low = cm & (~cm + 1u);
+            if ((j >> r) & 1) o |= low;
+            cm &= cm - 1u;
+        }
+        return tpart | o;
+    };
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         uint64_t base = 0;
         for (int o = 0; o < pd.n_outer; ++o) base |= ((tile >> o) & 1) << pd.oq[o];
-        auto gidx = [&](unsigned i) {
+        // global index of amplitude j of the current layout: the thread's
+        // part (scattered once per layout) | the register bits' qubits, which
+        // are the same for the whole workgroup (scalar registers)
+        auto gidx_j = [&](int j) {
             uint64_t g = base;
+            for (int b2 = 0; b2 < B; ++b2)
+                if ((tpart >> b2) & 1) g |= uint64_t(1) << pd.tq[b2];
+            unsigned cm = __builtin_amdgcn_readfirstlane(cur_mask);
 #pragma unroll
-            for (int j = 0; j < B; ++j) g |= uint64_t((i >> j) & 1) << pd.tq[j];
+            for (int r = 0; r < RB; ++r) {
+                const int bit = __builtin_amdgcn_readfirstlane(__ffs(cm) - 1);
+                if ((j >> r) & 1) g |= uint64_t(1) << pd.tq[bit];
+                cm &= cm - 1u;
+            }
             return g;
         };
         auto layout = [&](unsigned mask) {
             tpart = pdep_bits(t, kTileMask & ~mask);
             cur_mask = mask;
-#pragma unroll
-            for (int j = 0; j < NR; ++j) idx[j] = tpart | pdep_bits(unsigned(j), mask);
         };
         layout(first_mask);
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            const uint64_t g = gidx(idx[j]);
+            const uint64_t g = gidx_j(j);
             p[j] = psi[g];
             l[j] = lam[g];
         }
@@ -436,15 +455,15 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
                 __syncthreads();  // every thread has read the previous layout's LDS data
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
-                    LP[idx[j]] = p[j];
-                    LL[idx[j]] = l[j];
+                    LP[ix(j)] = p[j];
+                    LL[ix(j)] = l[j];
                 }
                 __syncthreads();
                 layout(st.mask);
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
-                    p[j] = LP[idx[j]];
-                    l[j] = LL[idx[j]];
+                    p[j] = LP[ix(j)];
+                    l[j] = LL[ix(j)];
                 }
                 break;
             }
@@ -516,7 +535,7 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
                 const cplx<T> d3 = st.k == 2 ? cvt<T>(mats[st.mat + 3 * (dim + 1)]) : d0;
 #pragma unroll
                 for (int j = 0; j < NR; ++j) {
-                    const unsigned i = idx[j];
+                    const unsigned i = ix(j);
                     const int row = st.k == 2 ? int(((i >> st.bits[0]) & 1) << 1 | ((i >> st.bits[1]) & 1))
                                               : int((i >> st.bits[0]) & 1);
                     const cplx<T> ph = row == 0 ? d0 : row == 1 ? d1 : row == 2 ? d2 : d3;
@@ -542,10 +561,13 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
 #pragma unroll
                         for (int j = 0; j < NR; ++j) {
                             const double v = tm.cre * wi[j] + tm.cim * wr[j];
-                            acc += (__popc(tm.z & idx[j]) & 1) ? -v : v;
+                            acc += (__popc(tm.z & ix(j)) & 1) ? -v : v;
                         }
                     }
                 } else {
+                    unsigned idx[NR];
+#pragma unroll
+                    for (int j = 0; j < NR; ++j) idx[j] = ix(j);
                     for (int q = 0; q < st.nterms; ++q) acc += r_grad_any<T, NR>(p, l, idx, terms[st.term0 + q]);
                 }
                 for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
@@ -556,7 +578,7 @@ __global__ __launch_bounds__(1 << (B - RB)) void k_adjoint_reg(cplx<T> *__restri
         }
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
-            const uint64_t g = gidx(idx[j]);
+            const uint64_t g = gidx_j(j);
             psi[g] = p[j];
             lam[g] = l[j];
         }
