@@ -194,3 +194,21 @@ def test_tiled_reverse_sweep_complex64():
     _, g64 = A.Estimator(pc, H, dtype="c128").gradient(th)
     _, g32 = A.Estimator(pc, H, dtype="c64").gradient(th)
     assert np.max(np.abs(g32 - g64)) <= 1e-4 * max(1.0, np.max(np.abs(g64)))
+
+
+@pytest.mark.parametrize("n", [9, 12])
+def test_tiled_reverse_sweep_with_three_qubit_gates(n):
+    """Passes holding a 3-qubit gate run LDS-resident, the others
+    register-resident; both agree with the per-parameter sweep."""
+    pc, H, th = wide_circuit(n, 40 + n, layers=2)
+    pc.ccx(0, n // 2, n - 1)
+    pc.ry(n - 1, param=pc.num_params)
+    pc.cswap(n - 2, 1, n // 2)
+    pc.rz(2, param=pc.num_params)
+    th = np.concatenate([th, [0.3, 1.1]])
+    est = A.Estimator(pc, H)
+    e_t, g_t = est.gradient(th)
+    est._eng.set_tiled_sweep(False)
+    e_p, g_p = est.gradient(th)
+    assert abs(e_t - e_p) <= 1e-13
+    assert np.max(np.abs(g_t - g_p)) <= 1e-12

@@ -296,3 +296,13 @@ def test_shots_profile_file(tmp_path, monkeypatch):
     for k in ("preprocessing_time", "shots_total_time", "total_time", "svd_time"):
         assert isinstance(data[k], float) and data[k] >= 0.0
     assert isinstance(data["svd_calls"], int) and data["svd_calls"] > 0
+
+
+def test_single_site_and_measure_everything():
+    c = C.circuit(1, [C.instruction("h", qubit=0), C.instruction("measure", qubit=0, cbit=0)], n_cbits=1)
+    text = c.model_dump_json()
+    for seed in range(6):
+        want, cb, _ = orc.simulate_mps_json(text, seed)
+        r = dq.MpsSimulator(seed=seed).simulate(c)
+        assert r.classical_bits == cb and np.max(np.abs(r.statevector - want)) <= 1e-15
+    assert sum(dq.MpsSimulator(seed=1).simulate_shots(c, shots=200).values()) == 200

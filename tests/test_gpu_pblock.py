@@ -124,3 +124,21 @@ def test_epoch_draws_and_unsupported_gates():
                        {0: [0], 1: [1]})
     with pytest.raises(ValueError, match="remote_rzz requires one theta parameter"):
         dq.PBlockSimulator(seed=1).simulate(bad)
+
+
+def test_empty_node_and_untouched_blocks():
+    """A node without qubits is a 0-qubit block; blocks no instruction merges
+    are tensored in node order at the end."""
+    q, c = {"q": C.Register("q", 5, 0)}, {"c": C.Register("c", 5, 0)}
+    ins = [C.instruction("h", qubit=0), C.instruction("ry", qubit=3, theta=0.7),
+           C.instruction("cx", control=0, target=1), C.instruction("measure", qubit=1, cbit=1)]
+    qpn = {0: [0, 1], 1: [], 2: [2, 3], 3: [4]}
+    d = C.distribute(C.Circuit(q, c, ins), {k: v for k, v in qpn.items() if v})
+    d.qubits_per_node = qpn
+    d.circuits[1] = C.Circuit(q, c, [])
+    for seed in (1, 2, 3):
+        st, qs, cb = orc.simulate_pblock(d, seed)
+        r = dq.PBlockSimulator(seed=seed).simulate(d)
+        assert r.physical_qubits == qs == [0, 1, 2, 3, 4]
+        assert r.classical_bits == cb
+        assert np.max(np.abs(r.statevector - st)) <= 1e-14
