@@ -380,7 +380,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only in this build)", file=sys.stderr)
+              "(the engine runs a state on one GPU; sharding across GPUs is not implemented)", file=sys.stderr)
         return 2
 
     import dqsim_bosonic_amd as dq  # noqa: F401  (loads the HIP engine)
