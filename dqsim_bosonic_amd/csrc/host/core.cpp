@@ -1998,6 +1998,40 @@ void reject_options(const py::kwargs &options, const std::string &mode) {
 
 }  // namespace
 
+// Plain op list for the planner bindings: (qubits, flat complex matrix,
+// control values or empty); with control values the matrix is the 2x2
+// applied to qubits[-1] under controls qubits[:-1] (as dq_apply_1q).
+static std::vector<dq::HostOp> host_ops(py::list ops) {
+    std::vector<dq::HostOp> queue;
+    for (py::handle item : ops) {
+        py::tuple t = py::reinterpret_borrow<py::tuple>(item);
+        auto qs = t[0].cast<std::vector<int>>();
+        auto u = t[1].cast<std::vector<std::complex<double>>>();
+        auto cv = t[2].cast<std::vector<int>>();
+        dq::HostOp op;
+        op.k = int(qs.size());
+        for (int j = 0; j < op.k; ++j) op.q[j] = qs[size_t(j)];
+        std::vector<double> flat(2 * u.size());
+        for (size_t e = 0; e < u.size(); ++e) {
+            flat[2 * e] = u[e].real();
+            flat[2 * e + 1] = u[e].imag();
+        }
+        if (!cv.empty()) {
+            op.kind = dq::K_CTRL1;
+            for (int j = 0; j + 1 < op.k; ++j) {
+                op.cmask |= uint64_t(1) << qs[size_t(j)];
+                if (cv[size_t(j)]) op.cval |= uint64_t(1) << qs[size_t(j)];
+            }
+            op.mat = flat;
+            op.dense = flat;
+        } else {
+            dq::classify(op, flat.data());
+        }
+        queue.push_back(std::move(op));
+    }
+    return queue;
+}
+
 PYBIND11_MODULE(_core, m) {
     m.doc() = "MI355X statevector engine: the reference's dqsim statevector API over HIP kernels";
 
@@ -2209,33 +2243,7 @@ PYBIND11_MODULE(_core, m) {
     // order, (qubits, 32x32 matrix, gate count) for a fused block or
     // (original index,) for an op kept as it is.
     m.def("_fuse_blocks", [](std::vector<int> tile_qubits, py::list ops, double min_cost, int max_qubits) {
-        std::vector<dq::HostOp> queue;
-        for (py::handle item : ops) {
-            py::tuple t = py::reinterpret_borrow<py::tuple>(item);
-            auto qs = t[0].cast<std::vector<int>>();
-            auto u = t[1].cast<std::vector<std::complex<double>>>();
-            auto cv = t[2].cast<std::vector<int>>();
-            dq::HostOp op;
-            op.k = int(qs.size());
-            for (int j = 0; j < op.k; ++j) op.q[j] = qs[size_t(j)];
-            std::vector<double> flat(2 * u.size());
-            for (size_t e = 0; e < u.size(); ++e) {
-                flat[2 * e] = u[e].real();
-                flat[2 * e + 1] = u[e].imag();
-            }
-            if (!cv.empty()) {
-                op.kind = dq::K_CTRL1;
-                for (int j = 0; j + 1 < op.k; ++j) {
-                    op.cmask |= uint64_t(1) << qs[size_t(j)];
-                    if (cv[size_t(j)]) op.cval |= uint64_t(1) << qs[size_t(j)];
-                }
-                op.mat = flat;
-                op.dense = flat;
-            } else {
-                dq::classify(op, flat.data());
-            }
-            queue.push_back(std::move(op));
-        }
+        std::vector<dq::HostOp> queue = host_ops(ops);
         int pos[64], tq[16] = {0};
         for (int q = 0; q < 64; ++q) pos[q] = dq::kOuterBit + q;
         for (size_t t = 0; t < tile_qubits.size() && t < 16; ++t) {
@@ -2263,6 +2271,22 @@ PYBIND11_MODULE(_core, m) {
         }
         return out;
     }, py::arg("tile_qubits"), py::arg("ops"), py::arg("min_cost") = 0.0, py::arg("max_qubits") = 5);
+    // The pass builder on a plain op list (as _fuse_blocks); returns, per
+    // pass, (tile qubit mask, op indices in the order the pass applies them).
+    m.def("_schedule", [](int n, py::list ops, int fuse, int tile_bits, int low_bits, bool reg_tiles) {
+        std::vector<dq::HostOp> queue = host_ops(ops);
+        dq::PlanConfig cfg;
+        cfg.n = n;
+        cfg.fuse = fuse;
+        cfg.tile_bits = tile_bits;
+        cfg.low_bits = low_bits;
+        cfg.reg_tiles = reg_tiles ? 1 : 0;
+        const bool anywhere = fuse && reg_tiles && dq::effective_tile_bits(&cfg) >= 9;
+        py::list out;
+        for (const dq::Pass &p : dq::schedule(&cfg, queue, anywhere)) out.append(py::make_tuple(p.S, p.ops));
+        return out;
+    }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
+       py::arg("reg_tiles") = true);
     m.def("_fuse_circuit", [](const std::string &text) {
         Circuit c = Decoder().decode(text);
         py::list out;

@@ -1,0 +1,133 @@
+"""CPU tests of the pass builder (no device needed).
+
+A pass is one HBM sweep over tiles of b qubits; the builder may move an op
+ahead of earlier ops only when they commute (disjoint qubits, or both
+diagonal).  Checks: every op lands in exactly one pass; each op's
+non-diagonal qubits are tile qubits of its pass (a controlled 2x2 needs
+only its target); applying the ops in pass order to a random state gives the
+program's state to FP64 rounding (oracle gate application), so a reordering
+across a shared qubit fails by O(1); and on layered circuits the tile choice
+needs no more passes than taking qubits in order of first use.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.test_fusion_plan import apply_op, random_ops
+
+core = pytest.importorskip("dqsim_bosonic_amd._core")
+
+
+def is_diag(u, cv):
+    if cv:
+        return False
+    d = np.asarray(u).reshape(int(np.sqrt(np.asarray(u).size)), -1)
+    return np.count_nonzero(d - np.diag(np.diag(d))) == 0
+
+
+def is_controlled(u):
+    """Identity outside the bottom-right 2x2 block: a 2x2 on qubits[-1]
+    under controls qubits[:-1] all 1 (the planner classifies it so)."""
+    d = np.asarray(u).reshape(int(np.sqrt(np.asarray(u).size)), -1)
+    if d.shape[0] < 4:
+        return False
+    e = d.copy()
+    e[-2:, -2:] = np.eye(2)
+    return np.array_equal(e, np.eye(d.shape[0]))
+
+
+def need_mask(qs, u, cv):
+    """Tile qubits an op needs, by the planner's classification order:
+    diagonal (none), then controlled 2x2 (the target), else all."""
+    if is_diag(u, cv):
+        return 0
+    if cv or is_controlled(u):
+        return 1 << qs[-1]
+    return sum(1 << q for q in qs)
+
+
+def layered_ops(n, depth, rng):
+    """Random 1q gate on every qubit (h/sx/ry dense, rz/t diagonal), then a
+    random pairing with diagonal 2q gates, per layer."""
+    ops = []
+    for _ in range(depth):
+        for q in range(n):
+            g = ["h", "sx", "ry", "rz", "t"][int(rng.integers(5))]
+            ops.append(([q], orc.gate_matrix(g, rng.uniform(0, 6, 1 if g in ("ry", "rz") else 0)), []))
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            ops.append(([int(perm[i]), int(perm[i + 1])], orc.gate_matrix("rzz", rng.uniform(0, 6, 1)), []))
+    return ops
+
+
+def first_use_passes(ops, n, b, low):
+    """Pass count when a pass takes tile qubits in order of first use (the
+    builder's rule at fusion level 1, with level 2's lookahead)."""
+    rem = list(range(len(ops)))
+    count = 0
+    while rem:
+        S = (1 << low) - 1
+        nd = dd = 0
+        rest = []
+        for i in rem:
+            qs, u, cv = ops[i]
+            qm = sum(1 << q for q in qs)
+            diag = is_diag(u, cv)
+            block = nd if diag else (nd | dd)
+            if qm & block:
+                if diag:
+                    dd |= qm
+                else:
+                    nd |= qm
+                rest.append(i)
+                continue
+            need = need_mask(qs, u, cv)
+            if bin(S | need).count("1") <= b:
+                S |= need
+            else:
+                nd |= qm
+                rest.append(i)
+        count += 1
+        rem = rest
+    return count
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("n,tile_bits,low_bits", [(12, 9, 4), (13, 9, 3), (14, 12, 4)])
+def test_passes_equal_program(seed, n, tile_bits, low_bits):
+    rng = np.random.default_rng(100 + seed)
+    ops = random_ops(n, 150, rng)
+    # a share of diagonal ops on arbitrary qubits (they may move past each other)
+    for _ in range(30):
+        qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+        ops.insert(int(rng.integers(len(ops))), (qs, orc.gate_matrix("cp", rng.uniform(-4, 4, 1)), []))
+    passes = core._schedule(n, [(qs, np.asarray(u).ravel(), cv) for qs, u, cv in ops], 2, tile_bits, low_bits)
+    seen = [i for _, idx in passes for i in idx]
+    assert sorted(seen) == list(range(len(ops)))
+    for S, idx in passes:
+        assert bin(S).count("1") == tile_bits
+        assert S & ((1 << low_bits) - 1) == (1 << low_bits) - 1
+        for i in idx:
+            qs, u, cv = ops[i]
+            assert need_mask(qs, u, cv) & ~S == 0, (i, qs, hex(S))
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    want = psi0.copy()
+    for qs, u, cv in ops:
+        want = apply_op(want, n, qs, u, cv)
+    got = psi0.copy()
+    for i in seen:
+        got = apply_op(got, n, *ops[i])
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("n,tile_bits,low_bits", [(20, 12, 4), (24, 12, 4), (24, 12, 3)])
+def test_tile_choice_beats_first_use(seed, n, tile_bits, low_bits):
+    rng = np.random.default_rng(7 + seed)
+    ops = layered_ops(n, 10, rng)
+    passes = core._schedule(n, [(qs, np.asarray(u).ravel(), cv) for qs, u, cv in ops], 2, tile_bits, low_bits)
+    assert sorted(i for _, idx in passes for i in idx) == list(range(len(ops)))
+    assert len(passes) <= first_use_passes(ops, n, tile_bits, low_bits)
