@@ -755,7 +755,14 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     const size_t real_b = sizeof(T);
     const size_t ops_b = align_up(std::max<size_t>(total_ops, 1) * sizeof(TileOp), 256);
     const size_t mats_b = align_up(std::max<size_t>(mats.size(), 2) * real_b, 256);
-    const size_t total = ops_b + mats_b;
+    // layout tables follow the matrix pool; each pass finds its own at pd.lay_off
+    std::vector<uint32_t> lays;
+    for (RegPlan &rp : plans) {
+        rp.pd.lay_off = int(mats_b + lays.size() * sizeof(uint32_t));
+        for (int rb : rp.layouts) append_layout_table(lays, rb, rp.b, unsigned(2 * sizeof(T)));
+    }
+    const size_t lays_b = align_up(std::max<size_t>(lays.size(), 1) * sizeof(uint32_t), 256);
+    const size_t total = ops_b + mats_b + lays_b;
     HIP_TRY(hipEventSynchronize(s->upload_done));
     if (int rc = ensure_host_buffer(s, total)) return rc;
     if (int rc = ensure_device_buffer(s, total)) return rc;
@@ -767,6 +774,7 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     }
     T *hm = reinterpret_cast<T *>(h + ops_b);
     for (size_t i = 0; i < mats.size(); ++i) hm[i] = T(mats[i]);
+    std::memcpy(h + ops_b + mats_b, lays.data(), lays.size() * sizeof(uint32_t));
     HIP_TRY(hipMemcpyAsync(s->d_upload, h, total, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipEventRecord(s->upload_done, s->stream));
     const TileOp *d_ops = reinterpret_cast<const TileOp *>(s->d_upload);

@@ -634,9 +634,29 @@ struct RegPlan {
     TilePass pd;
     int b;
     std::vector<TileOp> ops;
+    std::vector<int> layouts;  // distinct layouts of the pass (entries of its layout table)
     int fused_blocks = 0;  // K_FUSED ops formed in this pass
     int fused_gates = 0;   // gates they replace
 };
+
+// The kernel's LDS swizzle of a tile index (dq_tile.hpp: swz): the low nibble
+// XORed with every higher nibble of the b tile bits.
+inline unsigned tile_swizzle(unsigned i, int b) {
+    unsigned s = 0;
+    for (int sh = 4; sh < b; sh += 4) s ^= (i >> sh) & 15u;
+    return i ^ s;
+}
+
+// One layout-table entry: for register j < 16 of layout rb, the LDS byte
+// offset of its register bits, swizzled, for amplitudes of elem bytes.
+inline void append_layout_table(std::vector<uint32_t> &out, int rb, int b, unsigned elem) {
+    for (int j = 0; j < 16; ++j) {
+        unsigned v = 0;
+        for (int m = 0; m < 4; ++m)
+            if ((j >> m) & 1) v |= 1u << ((rb >> (4 * m)) & 15);
+        out.push_back(tile_swizzle(v, b) * elem);
+    }
+}
 
 // Plans the layout of every op of one pass and encodes the ops.
 inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, const Pass &p, std::vector<double> &mats,
@@ -737,6 +757,14 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     for (size_t i = 0; i < first && i < nops; ++i) lay[i] = pd.load_rb;
     const int last = nops ? lay[nops - 1] : pd.load_rb;
     pd.store_rb = (unpack_rb(last) & lowmask) ? pack_rb(dflt) : last;
+    auto lay_index = [&](int rb) {
+        for (size_t e = 0; e < out.layouts.size(); ++e)
+            if (out.layouts[e] == rb) return int(e);
+        out.layouts.push_back(rb);
+        return int(out.layouts.size() - 1);
+    };
+    pd.load_lay = lay_index(pd.load_rb);
+    pd.store_lay = lay_index(pd.store_rb);
 
     for (size_t i = 0; i < nops; ++i) {
         const HostOp &h = *eff[i];
@@ -745,6 +773,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
         t.k = h.k;
         t.relayout = relay[i];
         t.rb = lay[i];
+        t.lay = lay_index(lay[i]);
         for (int j = 0; j < h.k; ++j) t.bit[j] = pos[h.q[j]];
         for (int j = 0; j < h.k; ++j) t.sbit[j] = t.bit[j];
         std::sort(t.sbit, t.sbit + h.k);

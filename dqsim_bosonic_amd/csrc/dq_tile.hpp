@@ -283,19 +283,40 @@ __device__ __forceinline__ void dispatch_dense2(cplx<T> (&a)[16], int p, int q, 
 // The swizzle XORs nibbles together, so it is linear over XOR, and a tile
 // index is tpart | jp[j] with disjoint bits:
 //   swz(tpart | jp[j]) = swz(tpart) ^ swz(jp[j]),
-// one per-thread value per layout and 16 uniform ones.
+// one per-thread value per layout and 16 uniform ones.  The uniform ones come
+// from the host as byte offsets (the layout table of the pass, one entry of
+// 16 per layout), so each LDS access costs one XOR: computed in the kernel
+// they were folded back into a full per-lane swizzle of every index.
 template <typename T, int B>
-__device__ __forceinline__ void to_lds(cplx<T> *lds, const cplx<T> (&a)[16], const Layout<T> &L) {
-    const unsigned st = swz<B>(L.tpart);
-#pragma unroll
-    for (int j = 0; j < 16; ++j) lds[st ^ swz<B>(L.jp[j])] = a[j];
+__device__ __forceinline__ unsigned lds_base(const Layout<T> &L) {
+    return swz<B>(L.tpart) * unsigned(sizeof(cplx<T>));
 }
 
-template <typename T, int B>
-__device__ __forceinline__ void from_lds(cplx<T> (&a)[16], const cplx<T> *lds, const Layout<T> &L) {
-    const unsigned st = swz<B>(L.tpart);
+// An amplitude as one naturally aligned vector, so that each LDS access is a
+// single 2-element access (ds_write_b128 / ds_read_b128 for complex128).
+template <typename T>
+using amp_vec = T __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ void to_lds(unsigned char *lds, const cplx<T> (&a)[16], unsigned st,
+                                       const unsigned *__restrict__ sj) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) a[j] = lds[st ^ swz<B>(L.jp[j])];
+    for (int j = 0; j < 16; ++j) {
+        amp_vec<T> v;
+        v.x = a[j].re;
+        v.y = a[j].im;
+        *reinterpret_cast<amp_vec<T> *>(lds + (st ^ sj[j])) = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void from_lds(cplx<T> (&a)[16], const unsigned char *lds, unsigned st,
+                                         const unsigned *__restrict__ sj) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const amp_vec<T> v = *reinterpret_cast<const amp_vec<T> *>(lds + (st ^ sj[j]));
+        a[j] = cplx<T>{v.x, v.y};
+    }
 }
 
 // Diagonal with one register bit (slot P): registers with that bit set take
@@ -381,6 +402,9 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
     double red = 0.0;  // this thread's share of the epilogue reduction
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+    // layout tables of this pass: 16 LDS byte offsets per layout
+    const unsigned *laytab =
+        reinterpret_cast<const unsigned *>(reinterpret_cast<const unsigned char *>(mats) + pd.lay_off);
     const unsigned t = threadIdx.x;
     Layout<T> L;
     // global offsets of the load/store layouts: per-thread part + 4 uniform
@@ -440,15 +464,19 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             }
         }
         make_layout<T, B>(L, pd.load_rb, t);
+        unsigned st = lds_base<T, B>(L);
+        const unsigned *cs = laytab + 16 * pd.load_lay;
         int cur = pd.load_rb;
         for (int o = 0; o < pd.op_count; ++o) {
             const TileOp &op = ops[pd.op_begin + o];
             if (op.relayout) {
                 __syncthreads();
-                to_lds<T, B>(lds, a, L);
+                to_lds<T>(smem_raw, a, st, cs);
                 __syncthreads();
                 make_layout<T, B>(L, op.rb, t);
-                from_lds<T, B>(a, lds, L);
+                st = lds_base<T, B>(L);
+                cs = laytab + 16 * op.lay;
+                from_lds<T>(a, smem_raw, st, cs);
                 cur = op.rb;
             }
             const cplx<T> *m = mats + op.mat;
@@ -529,17 +557,17 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             case T_FUSED5:
                 if constexpr (sizeof(T) == 4 && B >= 12 && FAST == 2) {  // only the MFMA instantiation
                     __syncthreads();
-                    to_lds<T, B>(lds, a, L);
+                    to_lds<T>(smem_raw, a, st, cs);
                     __syncthreads();
                     fused5_mfma<B>(reinterpret_cast<cplx<float> *>(lds), reinterpret_cast<const cplx<float> *>(m),
                                    op, t);
                     __syncthreads();
-                    from_lds<T, B>(a, lds, L);
+                    from_lds<T>(a, smem_raw, st, cs);
                 }
                 break;
             default: {  // T_LDSOP
                 __syncthreads();
-                to_lds<T, B>(lds, a, L);
+                to_lds<T>(smem_raw, a, st, cs);
                 __syncthreads();
                 const int k = op.k;
                 const int dim = 1 << k;
@@ -567,10 +595,10 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
         }
         if (cur != pd.store_rb) {
             __syncthreads();
-            to_lds<T, B>(lds, a, L);
+            to_lds<T>(smem_raw, a, st, cs);
             __syncthreads();
             make_layout<T, B>(L, pd.store_rb, t);
-            from_lds<T, B>(a, lds, L);
+            from_lds<T>(a, smem_raw, lds_base<T, B>(L), laytab + 16 * pd.store_lay);
         }
         if (pd.red_bit >= 0) {
             const int rb2 = pd.red_bit;

@@ -63,6 +63,8 @@ struct TileOp {
     unsigned ocmask, ocval;  // T_CTRL1 controls on outer qubits (outer-index bits)
     int nsel;      // T_DENSE1/T_DENSE2: > 0 selects matrix number sum_j bit(sel[j]) << j
     int sel[3];    // outer-index bits of the selector
+    int lay;       // entry of layout `rb` in the pass's layout table
+    int pad;
     double norm;
 };
 
@@ -77,6 +79,10 @@ struct TilePass {
     int op_count;
     int load_rb;   // layout of the global load
     int store_rb;  // layout of the global store
+    int lay_off;   // byte offset, from the matrix pool, of the pass's layout table:
+                   // per layout, the LDS byte offsets swz(register bits of j) * amplitude size, j < 16
+    int load_lay;  // table entries of load_rb and store_rb
+    int store_lay;
     int tq[16];    // tile bit -> qubit
     int oq[48];    // outer bit -> qubit
 };
