@@ -710,7 +710,10 @@ template <typename T>
 int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *mats) {
     SweepTimer timer(s);
     const uint64_t ntiles = uint64_t(1) << rp.pd.n_outer;
+    // a power of two: the kernel derives each tile's offset from blockIdx.x
+    // and the tile's step count
     const uint64_t grid = std::min<uint64_t>(ntiles, 16384);
+    if (grid & (grid - 1)) return fail(DQ_ERR_ARG, "tile grid is not a power of two");
     // fusion level 1 keeps the reference's rounding: the exact kernel;
     // level 2 already reorders commuting ops and uses the contracted kernel.
     auto *fn = std::is_same<T, double>::value

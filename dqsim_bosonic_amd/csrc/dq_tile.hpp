@@ -422,16 +422,24 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             gs_r[mm] = uint64_t(1) << pd.tq[(pd.store_rb >> (4 * mm)) & 15];
         }
     }
-    auto tile_base = [&](uint64_t tile) {
-        uint64_t base = 0;
-        for (int j = 0; j < pd.n_outer; ++j)
-            if ((tile >> j) & 1) base |= uint64_t(1) << pd.oq[j];
-        return base;
-    };
+    // Global offset of a tile: its outer-index bits deposited on the outer
+    // qubits (oq ascending).  Tiles are blockIdx.x + k * gridDim.x with
+    // gridDim.x a power of two (the host launches it so), so the offset is
+    // the deposit of blockIdx.x, fixed per workgroup, ORed with the deposit
+    // of k on the outer qubits of the high index bits (hmask); that one steps
+    // from k to k + 1 as a counter confined to the bits of hmask.
+    uint64_t dep_lo = 0, hmask = 0;
+    for (int j = 0; j < pd.n_outer; ++j) {
+        const uint64_t qb = uint64_t(1) << pd.oq[j];
+        if ((uint64_t(blockIdx.x) >> j) & 1) dep_lo |= qb;
+        if (!(((uint64_t(gridDim.x) - 1) >> j) & 1)) hmask |= qb;
+    }
+    auto next_high = [&](uint64_t h) { return ((h | ~hmask) + 1) & hmask; };
     cplx<T> nxt[16];
     uint64_t tile = blockIdx.x;
+    uint64_t high = 0;  // deposit of the current tile's high index bits
     if (tile < ntiles) {
-        const uint64_t base = tile_base(tile) + g_t;
+        const uint64_t base = dep_lo + g_t;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             uint64_t off = base;
@@ -447,13 +455,13 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
     // both sides (the next tile's loads, then this tile's stores); the wait
     // for the loads then leaves the stores in flight.
     auto process = [&](uint64_t tile) __attribute__((always_inline)) {
-        const uint64_t base = tile_base(tile);
+        const uint64_t base = dep_lo | high;
         cplx<T> a[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) a[j] = nxt[j];
         const uint64_t next = tile + gridDim.x;
         if (next < ntiles) {
-            const uint64_t nb = tile_base(next) + g_t;
+            const uint64_t nb = (dep_lo | next_high(high)) + g_t;
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 uint64_t off = nb;
@@ -482,7 +490,9 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             const cplx<T> *m = mats + op.mat;
             if (op.nsel > 0) {  // one matrix per value of the selector bits (constant over the tile)
                 unsigned e = 0;
-                for (int j = 0; j < op.nsel; ++j) e |= unsigned((tile >> op.sel[j]) & 1) << j;
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (j < op.nsel) e |= unsigned((tile >> op.sel[j]) & 1) << j;
                 m += e * (op.kind == T_DENSE2 ? 16u : 4u);
             }
             switch (op.kind) {
@@ -500,7 +510,9 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 // e = thread part | register part; the thread part picks the
                 // candidate entries once per op (uniform values, no gathers).
                 int et = 0;
-                for (int b2 = 0; b2 < op.k; ++b2) {
+#pragma unroll
+                for (int b2 = 0; b2 < 2; ++b2) {
+                    if (b2 >= op.k) break;
                     const int bp = op.bit[b2];
                     if (bp >= kOuterBit) et |= int((tile >> (bp - kOuterBit)) & 1) << (op.k - 1 - b2);
                     else if (b2 != op.q || op.p < 0) et |= int((L.tpart >> bp) & 1) << (op.k - 1 - b2);
@@ -624,7 +636,10 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
     };
     if (tile < ntiles) {
         process(tile);
-        for (tile += gridDim.x; tile < ntiles; tile += gridDim.x) process(tile);
+        for (tile += gridDim.x; tile < ntiles; tile += gridDim.x) {
+            high = next_high(high);
+            process(tile);
+        }
     }
     if (pd.red_bit >= 0) {
         // fixed-shape block sum: butterflies within each group of kGroup
