@@ -51,13 +51,14 @@ __device__ __forceinline__ void make_layout(Layout<T> &L, int rb, unsigned t) {
             if ((j >> m) & 1) v |= 1u << rbit[m];
         L.jp[j] = v;
     }
-    unsigned tp = 0;
-    int c = 0;
+    // thread bit c -> the c-th non-register tile bit, ascending: insert a
+    // zero bit into t at each register bit, in ascending order (the host
+    // packs the register bits of a layout ascending)
+    unsigned tp = t;
 #pragma unroll
-    for (int bpos = 0; bpos < B; ++bpos) {
-        if ((rmask >> bpos) & 1) continue;
-        if ((t >> c) & 1) tp |= 1u << bpos;
-        ++c;
+    for (int m = 0; m < 4; ++m) {
+        const unsigned lo = (1u << rbit[m]) - 1;
+        tp = (tp & lo) | ((tp & ~lo) << 1);
     }
     L.tpart = tp;
 }
