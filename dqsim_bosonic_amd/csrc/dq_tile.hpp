@@ -73,6 +73,32 @@ __device__ __forceinline__ uint64_t gmap(unsigned i, const TilePass &pd) {
     return g;
 }
 
+// The dispatch header of an op (the first 12 words of TileOp), read as one
+// block so that the next op's header can be fetched while this op computes.
+struct OpHead {
+    int kind, relayout, rb, p, q, mat, variant, nsel, lay, sel[3];
+};
+static_assert(offsetof(TileOp, sel) == 9 * sizeof(int) && offsetof(TileOp, k) == 12 * sizeof(int),
+              "TileOp must start with its 12-word dispatch header");
+
+__device__ __forceinline__ OpHead op_head(const TileOp *op) {
+    const int *w = reinterpret_cast<const int *>(op);
+    OpHead h;
+    h.kind = w[0];
+    h.relayout = w[1];
+    h.rb = w[2];
+    h.p = w[3];
+    h.q = w[4];
+    h.mat = w[5];
+    h.variant = w[6];
+    h.nsel = w[7];
+    h.lay = w[8];
+    h.sel[0] = w[9];
+    h.sel[1] = w[10];
+    h.sel[2] = w[11];
+    return h;
+}
+
 // ---- register ops ----
 
 template <typename T>
@@ -476,36 +502,39 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
         unsigned st = lds_base<T, B>(L);
         const unsigned *cs = laytab + 16 * pd.load_lay;
         int cur = pd.load_rb;
+        OpHead h = op_head(ops + pd.op_begin);
         for (int o = 0; o < pd.op_count; ++o) {
             const TileOp &op = ops[pd.op_begin + o];
-            if (op.relayout) {
+            // the next op's header is fetched now, used next iteration
+            const OpHead hn = op_head(ops + pd.op_begin + (o + 1 < pd.op_count ? o + 1 : o));
+            if (h.relayout) {
                 __syncthreads();
                 to_lds<T>(smem_raw, a, st, cs);
                 __syncthreads();
-                make_layout<T, B>(L, op.rb, t);
+                make_layout<T, B>(L, h.rb, t);
                 st = lds_base<T, B>(L);
-                cs = laytab + 16 * op.lay;
+                cs = laytab + 16 * h.lay;
                 from_lds<T>(a, smem_raw, st, cs);
-                cur = op.rb;
+                cur = h.rb;
             }
-            const cplx<T> *m = mats + op.mat;
-            if (op.nsel > 0) {  // one matrix per value of the selector bits (constant over the tile)
+            const cplx<T> *m = mats + h.mat;
+            if (h.nsel > 0) {  // one matrix per value of the selector bits (constant over the tile)
                 unsigned e = 0;
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
-                    if (j < op.nsel) e |= unsigned((tile >> op.sel[j]) & 1) << j;
-                m += e * (op.kind == T_DENSE2 ? 16u : 4u);
+                    if (j < h.nsel) e |= unsigned((tile >> h.sel[j]) & 1) << j;
+                m += e * (h.kind == T_DENSE2 ? 16u : 4u);
             }
-            switch (op.kind) {
+            switch (h.kind) {
             case T_DENSE1:
-                dispatch_dense1<T, FAST>(a, op.p, op.variant, m);
+                dispatch_dense1<T, FAST>(a, h.p, h.variant, m);
                 break;
             case T_CTRL1:
                 if ((unsigned(tile) & op.ocmask) == op.ocval)
-                    dispatch_ctrl1<T, FAST>(a, op.p, op.variant, m, L, op.cmask, op.cval);
+                    dispatch_ctrl1<T, FAST>(a, h.p, h.variant, m, L, op.cmask, op.cval);
                 break;
             case T_DENSE2:
-                dispatch_dense2<T, FAST>(a, op.p, op.q, op.variant, m);
+                dispatch_dense2<T, FAST>(a, h.p, h.q, h.variant, m);
                 break;
             case T_DIAGT: {
                 // e = thread part | register part; the thread part picks the
@@ -516,18 +545,18 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                     if (b2 >= op.k) break;
                     const int bp = op.bit[b2];
                     if (bp >= kOuterBit) et |= int((tile >> (bp - kOuterBit)) & 1) << (op.k - 1 - b2);
-                    else if (b2 != op.q || op.p < 0) et |= int((L.tpart >> bp) & 1) << (op.k - 1 - b2);
+                    else if (b2 != h.q || h.p < 0) et |= int((L.tpart >> bp) & 1) << (op.k - 1 - b2);
                 }
                 const cplx<T> m0 = m[0], m1 = m[1];
                 const cplx<T> m2 = op.k == 2 ? m[2] : m0, m3 = op.k == 2 ? m[3] : m1;
-                if (op.p < 0) {
+                if (h.p < 0) {
                     const cplx<T> d = pick4(et, m0, m1, m2, m3);
 #pragma unroll
                     for (int j = 0; j < 16; ++j) a[j] = cmul(d, a[j]);
                 } else {
-                    const int rw = 1 << (op.k - 1 - op.q);  // weight of the register bit in e
+                    const int rw = 1 << (op.k - 1 - h.q);  // weight of the register bit in e
                     const cplx<T> d0 = pick4(et, m0, m1, m2, m3), d1 = pick4(et | rw, m0, m1, m2, m3);
-                    switch (op.p) {
+                    switch (h.p) {
                     case 0: reg_diag_split<T, 0>(a, d0, d1); break;
                     case 1: reg_diag_split<T, 1>(a, d0, d1); break;
                     case 2: reg_diag_split<T, 2>(a, d0, d1); break;
@@ -605,6 +634,7 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 break;
             }
             }
+            h = hn;
         }
         if (cur != pd.store_rb) {
             __syncthreads();

@@ -49,21 +49,23 @@ enum TileKind : int {
 enum OpVariant : int { V_GEN = 0, V_REAL = 1, V_DIAG = 2, V_ANTI = 3 };
 
 struct TileOp {
+    // dispatch header: the first 12 words, read as one block by the kernel
     int kind;
-    int k;
     int relayout;  // 1: change to layout `rb` before this op
     int rb;        // layout active from this op: 4 tile bits, 4 bits each, ascending
     int p, q;      // register slots (see TileKind)
+    int mat;       // offset (complex entries) into the matrix pool
+    int variant;
+    int nsel;      // T_DENSE1/T_DENSE2: > 0 selects matrix number sum_j bit(sel[j]) << j
+    int lay;       // entry of layout `rb` in the pass's layout table
+    int sel[3];    // outer-index bits of the selector
+    // operands
+    int k;
     int bit[5];    // tile bits, bit[0] = MSB of the gate index
     int sbit[5];   // ascending
     unsigned cmask, cval;
-    int mat;       // offset (complex entries) into the matrix pool
     int outcome;
-    int variant;
     unsigned ocmask, ocval;  // T_CTRL1 controls on outer qubits (outer-index bits)
-    int nsel;      // T_DENSE1/T_DENSE2: > 0 selects matrix number sum_j bit(sel[j]) << j
-    int sel[3];    // outer-index bits of the selector
-    int lay;       // entry of layout `rb` in the pass's layout table
     int pad;
     double norm;
 };
