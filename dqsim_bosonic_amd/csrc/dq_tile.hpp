@@ -22,8 +22,17 @@ namespace dq {
 
 template <typename T>
 struct Layout {
-    unsigned tpart;    // tile-index bits contributed by this thread
-    unsigned jp[16];   // tile-index bits contributed by register j (uniform)
+    unsigned tpart;  // tile-index bits contributed by this thread
+    int rb;          // the layout (register bits, 4 bits each, ascending)
+    // tile-index bits contributed by register j (uniform), computed where an
+    // op needs them rather than held for the whole pass
+    __device__ __forceinline__ unsigned jp(int j) const {
+        unsigned v = 0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            if ((j >> m) & 1) v |= 1u << ((rb >> (4 * m)) & 15);
+        return v;
+    }
 };
 
 template <int B>
@@ -37,20 +46,9 @@ __device__ __forceinline__ unsigned swz(unsigned i) {
 template <typename T, int B>
 __device__ __forceinline__ void make_layout(Layout<T> &L, int rb, unsigned t) {
     int rbit[4];
-    unsigned rmask = 0;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        rbit[m] = (rb >> (4 * m)) & 15;
-        rmask |= 1u << rbit[m];
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        unsigned v = 0;
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-            if ((j >> m) & 1) v |= 1u << rbit[m];
-        L.jp[j] = v;
-    }
+    for (int m = 0; m < 4; ++m) rbit[m] = (rb >> (4 * m)) & 15;
+    L.rb = rb;
     // thread bit c -> the c-th non-register tile bit, ascending: insert a
     // zero bit into t at each register bit, in ascending order (the host
     // packs the register bits of a layout ascending)
@@ -197,7 +195,7 @@ __device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[16], const cplx<T> *__res
             n0 = cadd(cmul(u00, x), cmul(u01, y));
             n1 = cadd(cmul(u10, x), cmul(u11, y));
         }
-        const bool on = ((L.tpart | L.jp[j0]) & cmask) == cval;
+        const bool on = ((L.tpart | L.jp(j0)) & cmask) == cval;
         a[j0] = on ? n0 : x;
         a[j1] = on ? n1 : y;
     }
@@ -568,7 +566,7 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             case T_DIAG:
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
-                    const unsigned i = L.tpart | L.jp[j];
+                    const unsigned i = L.tpart | L.jp(j);
                     int e = 0;
                     for (int b2 = 0; b2 < op.k; ++b2) {
                         const int bp = op.bit[b2];
@@ -585,7 +583,7 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 const int ov = outer ? int((tile >> (bp - kOuterBit)) & 1) : 0;
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
-                    const unsigned i = L.tpart | L.jp[j];
+                    const unsigned i = L.tpart | L.jp(j);
                     const int v = outer ? ov : int((i >> (bp & 31)) & 1);
                     if (v == op.outcome) {
                         a[j].re = T(double(a[j].re) / nrm);
@@ -617,7 +615,7 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                 for (int b2 = 0; b2 < k; ++b2) gm |= 1u << op.bit[b2];
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
-                    const unsigned i = L.tpart | L.jp[j];
+                    const unsigned i = L.tpart | L.jp(j);
                     int r = 0;
                     for (int b2 = 0; b2 < k; ++b2) r = (r << 1) | int((i >> op.bit[b2]) & 1);
                     const unsigned base_i = i & ~gm;
@@ -649,7 +647,7 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             const bool ov = outer && ((tile >> (rb2 - kOuterBit)) & 1);
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                const bool on = outer ? ov : (((L.tpart | L.jp[j]) >> (rb2 & 31)) & 1);
+                const bool on = outer ? ov : (((L.tpart | L.jp(j)) >> (rb2 & 31)) & 1);
                 const double re = double(a[j].re), im = double(a[j].im);
                 if (on) red += re * re + im * im;
             }
