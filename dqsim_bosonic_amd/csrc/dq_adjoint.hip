@@ -418,23 +418,42 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
         }
         return tpart | o;
     };
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint64_t base = 0;
-        for (int o = 0; o < pd.n_outer; ++o) base |= ((tile >> o) & 1) << pd.oq[o];
-        // global index of amplitude j of the current layout: the thread's
-        // part (scattered once per layout) | the register bits' qubits, which
-        // are the same for the whole workgroup (scalar registers)
-        auto gidx_j = [&](int j) {
-            uint64_t g = base;
+    // Global offset of a tile: its outer-index bits deposited on the outer
+    // qubits (oq ascending).  Tiles are blockIdx.x + k * gridDim.x with
+    // gridDim.x a power of two (the host launches it so): the deposit of
+    // blockIdx.x is fixed per workgroup, and that of k steps as a counter
+    // confined to the outer qubits of the high index bits (hmask).
+    uint64_t dep_lo = 0, hmask = 0;
+    for (int o = 0; o < pd.n_outer; ++o) {
+        const uint64_t qb = uint64_t(1) << pd.oq[o];
+        if ((uint64_t(blockIdx.x) >> o) & 1) dep_lo |= qb;
+        if (!(((uint64_t(gridDim.x) - 1) >> o) & 1)) hmask |= qb;
+    }
+    uint64_t high = 0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, high = ((high | ~hmask) + 1) & hmask) {
+        const uint64_t base = dep_lo | high;
+        // global offsets of the current layout: base | the thread's part,
+        // and the qubit offset of each register bit (scalar); amplitude j is
+        // the first ORed with the offsets of j's register bits
+        uint64_t gt = 0, gr[RB];
+        auto gmap_layout = [&]() {
+            gt = base;
+#pragma unroll
             for (int b2 = 0; b2 < B; ++b2)
-                if ((tpart >> b2) & 1) g |= uint64_t(1) << pd.tq[b2];
+                if ((tpart >> b2) & 1) gt |= uint64_t(1) << pd.tq[b2];
             unsigned cm = __builtin_amdgcn_readfirstlane(cur_mask);
 #pragma unroll
             for (int r = 0; r < RB; ++r) {
                 const int bit = __builtin_amdgcn_readfirstlane(__ffs(cm) - 1);
-                if ((j >> r) & 1) g |= uint64_t(1) << pd.tq[bit];
+                gr[r] = uint64_t(1) << pd.tq[bit];
                 cm &= cm - 1u;
             }
+        };
+        auto gidx_j = [&](int j) {
+            uint64_t g = gt;
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+                if ((j >> r) & 1) g |= gr[r];
             return g;
         };
         auto layout = [&](unsigned mask) {
@@ -442,6 +461,7 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
             cur_mask = mask;
         };
         layout(first_mask);
+        gmap_layout();
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             const uint64_t g = gidx_j(j);
@@ -576,6 +596,7 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
             }
             }
         }
+        gmap_layout();
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             const uint64_t g = gidx_j(j);
@@ -1032,7 +1053,9 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
         if (!rterms.empty())
             ADJ_TRY(hipMemcpyAsync(d_rterms, rterms.data(), rterms.size() * sizeof(AdjRTerm), hipMemcpyHostToDevice,
                                    stream));
+        // a power of two: the register kernel derives tile offsets from it
         const unsigned grid = unsigned(std::min<uint64_t>(ntiles, 8192));
+        if (grid & (grid - 1)) return afail(DQ_ERR_ARG, "grid is not a power of two");
         size_t slot_base = 0;
         for (size_t p = 0; p < hpasses.size(); ++p) {
             hipError_t e;
