@@ -316,6 +316,31 @@ struct AdjRTerm {
     double cre, cim;   // coefficient times i^{#Y}
 };
 
+// One DPP lane move of a double (two 32-bit moves); lanes that the row mask
+// leaves out read 0.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b & 0xffffffff), kCtrl, kRowMask, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), kCtrl, kRowMask, 0xf, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// Sum of v over the 64 lanes of a wave, complete in lane 63, in a fixed
+// order, with lane moves only (no LDS): pairs and quads by quad permutes,
+// rows of 16 by rotations by 4 and 8, then row 0 into row 1 and row 2 into
+// row 3 (broadcast of lane 15 of each row), then rows 0-1 into row 3
+// (broadcast of lane 31).
+__device__ __forceinline__ double wave_sum_lane63(double v) {
+    v += dpp_f64<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x124, 0xf>(v);  // row_ror:4
+    v += dpp_f64<0x128, 0xf>(v);  // row_ror:8
+    v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 __device__ __forceinline__ unsigned pdep_bits(unsigned v, unsigned mask) {
     unsigned out = 0;
     for (int b = 0; mask; ++b) {
@@ -590,8 +615,8 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
                     for (int j = 0; j < NR; ++j) idx[j] = ix(j);
                     for (int q = 0; q < st.nterms; ++q) acc += r_grad_any<T, NR>(p, l, idx, terms[st.term0 + q]);
                 }
-                for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-                if (lane == 0) partials[(uint64_t(st.slot) * ntiles + tile) * W + wave] = acc;
+                acc = wave_sum_lane63(acc);
+                if (lane == 63) partials[(uint64_t(st.slot) * ntiles + tile) * W + wave] = acc;
                 break;
             }
             }
