@@ -341,15 +341,9 @@ __device__ __forceinline__ double wave_sum_lane63(double v) {
     return v;
 }
 
-__device__ __forceinline__ unsigned pdep_bits(unsigned v, unsigned mask) {
-    unsigned out = 0;
-    for (int b = 0; mask; ++b) {
-        const unsigned low = mask & (~mask + 1u);
-        if ((v >> b) & 1) out |= low;
-        mask &= mask - 1u;
-    }
-    return out;
-}
+// An amplitude as one naturally aligned vector: one LDS access each.
+template <typename T>
+using amp2 = T __attribute__((ext_vector_type(2)));
 
 template <typename T, int NR, int M>
 __device__ __forceinline__ void r_dense1(cplx<T> (&a)[NR], const cplx<T> u0, const cplx<T> u1, const cplx<T> u2,
@@ -433,7 +427,7 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
     unsigned tpart = 0, cur_mask = 0;  // this thread's tile bits and the register bits of the layout
     // tile index of amplitude j: the thread's bits | j's register bits (the
     // layout is the same for the whole workgroup, so the latter is scalar)
-    auto ix = [&](int j) {
+    auto reg_bits = [&](int j) {  // j's register bits of the current layout (scalar)
         unsigned cm = __builtin_amdgcn_readfirstlane(cur_mask), o = 0;
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
@@ -441,7 +435,19 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
             if ((j >> r) & 1) o |= low;
             cm &= cm - 1u;
         }
-        return tpart | o;
+        return o;
+    };
+    auto ix = [&](int j) { return tpart | reg_bits(j); };
+    // LDS slot of tile index i during a relayout: the low nibble XORed with
+    // the higher nibbles, against bank conflicts of strided layouts.  It is
+    // linear over XOR, so slot(tpart | o) = slot(tpart) ^ slot(o): one
+    // per-thread value per layout and a scalar one per register, kept scalar
+    // (readfirstlane) so that each access is a single XOR.
+    auto slot = [](unsigned i) {
+        unsigned f = 0;
+#pragma unroll
+        for (int sh = 4; sh < B; sh += 4) f ^= (i >> sh) & 15u;
+        return i ^ f;
     };
     // Global offset of a tile: its outer-index bits deposited on the outer
     // qubits (oq ascending).  Tiles are blockIdx.x + k * gridDim.x with
@@ -481,8 +487,17 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
                 if ((j >> r) & 1) g |= gr[r];
             return g;
         };
+        // the thread's tile bits: t deposited on the non-register bits,
+        // ascending, i.e. a zero bit inserted into t at each register bit
         auto layout = [&](unsigned mask) {
-            tpart = pdep_bits(t, kTileMask & ~mask);
+            unsigned tp = t, cm = __builtin_amdgcn_readfirstlane(mask);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                const unsigned lo = (cm & (~cm + 1u)) - 1u;
+                tp = (tp & lo) | ((tp & ~lo) << 1);
+                cm &= cm - 1u;
+            }
+            tpart = tp;
             cur_mask = mask;
         };
         layout(first_mask);
@@ -498,17 +513,27 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
             switch (st.kind) {
             case R_RELAYOUT: {
                 __syncthreads();  // every thread has read the previous layout's LDS data
+                {
+                    const unsigned st_t = slot(tpart);
 #pragma unroll
-                for (int j = 0; j < NR; ++j) {
-                    LP[ix(j)] = p[j];
-                    LL[ix(j)] = l[j];
+                    for (int j = 0; j < NR; ++j) {
+                        const unsigned a = st_t ^ __builtin_amdgcn_readfirstlane(slot(reg_bits(j)));
+                        reinterpret_cast<amp2<T> *>(LP)[a] = amp2<T>{p[j].re, p[j].im};
+                        reinterpret_cast<amp2<T> *>(LL)[a] = amp2<T>{l[j].re, l[j].im};
+                    }
                 }
                 __syncthreads();
                 layout(st.mask);
+                {
+                    const unsigned st_t = slot(tpart);
 #pragma unroll
-                for (int j = 0; j < NR; ++j) {
-                    p[j] = LP[ix(j)];
-                    l[j] = LL[ix(j)];
+                    for (int j = 0; j < NR; ++j) {
+                        const unsigned a = st_t ^ __builtin_amdgcn_readfirstlane(slot(reg_bits(j)));
+                        const amp2<T> vp = reinterpret_cast<const amp2<T> *>(LP)[a];
+                        const amp2<T> vl = reinterpret_cast<const amp2<T> *>(LL)[a];
+                        p[j] = cplx<T>{vp.x, vp.y};
+                        l[j] = cplx<T>{vl.x, vl.y};
+                    }
                 }
                 break;
             }
