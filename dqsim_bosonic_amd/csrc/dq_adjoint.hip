@@ -617,7 +617,8 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
             default: {  // R_GRAD
                 double acc = 0.0;
                 if (st.a) {
-                    // every term diagonal (Z strings): Im(conj(l) c s p) = s (cre Im(conj(l) p) + cim Re(conj(l) p))
+                    // every term diagonal (Z strings): Im(conj(l) c s p) = s (cre Im(conj(l) p) + cim Re(conj(l) p)),
+                    // with the sign s split into a per-lane and a per-register (scalar) factor
                     double wr[NR], wi[NR];
 #pragma unroll
                     for (int j = 0; j < NR; ++j) {
@@ -628,11 +629,15 @@ __global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__res
                     }
                     for (int q = 0; q < st.nterms; ++q) {
                         const AdjRTerm tm = terms[st.term0 + q];
+                        double im_sum = 0.0, re_sum = 0.0;
 #pragma unroll
                         for (int j = 0; j < NR; ++j) {
-                            const double v = tm.cre * wi[j] + tm.cim * wr[j];
-                            acc += (__popc(tm.z & ix(j)) & 1) ? -v : v;
+                            const double sj = (__popc(tm.z & reg_bits(j)) & 1) ? -1.0 : 1.0;  // scalar
+                            im_sum += sj * wi[j];
+                            re_sum += sj * wr[j];
                         }
+                        const double lane_sign = (__popc(tm.z & tpart) & 1) ? -1.0 : 1.0;
+                        acc += lane_sign * (tm.cre * im_sum + tm.cim * re_sum);
                     }
                 } else {
                     unsigned idx[NR];
