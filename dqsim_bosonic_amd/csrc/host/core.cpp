@@ -397,6 +397,14 @@ PYBIND11_MODULE(_core, m) {
         return out;
     }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
        py::arg("reg_tiles") = true);
+    // One layout-table entry of the register sweep (dq_plan.hpp): the 16 LDS
+    // byte offsets of layout rb for tiles of b qubits and amplitudes of elem bytes.
+    m.def("_layout_table", [](int rb, int b, unsigned elem) {
+        std::vector<uint32_t> out;
+        dq::append_layout_table(out, rb, b, elem);
+        return out;
+    }, py::arg("rb"), py::arg("b"), py::arg("elem"));
+    m.def("_tile_swizzle", &dq::tile_swizzle, py::arg("i"), py::arg("b"));
     m.def("_fuse_circuit", [](const std::string &text) {
         Circuit c = Decoder().decode(text);
         py::list out;

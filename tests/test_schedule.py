@@ -131,3 +131,47 @@ def test_tile_choice_beats_first_use(seed, n, tile_bits, low_bits):
     passes = core._schedule(n, [(qs, np.asarray(u).ravel(), cv) for qs, u, cv in ops], 2, tile_bits, low_bits)
     assert sorted(i for _, idx in passes for i in idx) == list(range(len(ops)))
     assert len(passes) <= first_use_passes(ops, n, tile_bits, low_bits)
+
+
+def brute_swizzle(i, b):
+    """The kernel's LDS slot of tile index i: the low nibble XORed with each
+    higher nibble of the b tile bits."""
+    s = 0
+    sh = 4
+    while sh < b:
+        s ^= (i >> sh) & 15
+        sh += 4
+    return i ^ s
+
+
+@pytest.mark.parametrize("b", [9, 12, 13])
+def test_swizzle_is_a_permutation_linear_over_xor(b):
+    idx = np.arange(1 << b)
+    slots = np.array([core._tile_swizzle(int(i), b) for i in idx])
+    assert sorted(slots) == list(idx)
+    assert all(core._tile_swizzle(int(i), b) == brute_swizzle(int(i), b) for i in idx[:: 37])
+    rng = np.random.default_rng(b)
+    for _ in range(200):
+        x, y = (int(v) for v in rng.integers(1 << b, size=2))
+        assert core._tile_swizzle(x ^ y, b) == core._tile_swizzle(x, b) ^ core._tile_swizzle(y, b)
+
+
+@pytest.mark.parametrize("b,elem", [(9, 16), (12, 16), (12, 8), (13, 16)])
+def test_layout_table_matches_register_bits(b, elem):
+    """Entry j of a layout's table is the swizzled slot of j's register bits
+    (register slot m -> tile bit rbit[m], packed ascending) times the
+    amplitude size: slots within the tile, distinct over j, and XOR-linear, so
+    that swz(thread bits) ^ table[j] is the slot of the full index."""
+    rng = np.random.default_rng(b * elem)
+    for _ in range(20):
+        rbits = sorted(int(v) for v in rng.choice(b, size=4, replace=False))
+        rb = sum(q << (4 * m) for m, q in enumerate(rbits))
+        table = core._layout_table(rb, b, elem)
+        assert len(table) == 16
+        for j in range(16):
+            v = sum(1 << rbits[m] for m in range(4) if (j >> m) & 1)
+            assert table[j] == brute_swizzle(v, b) * elem
+        assert len(set(table)) == 16 and max(table) < (elem << b)
+        for j1 in range(16):
+            for j2 in range(16):
+                assert table[j1 ^ j2] == table[j1] ^ table[j2]
