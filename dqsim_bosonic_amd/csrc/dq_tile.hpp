@@ -5,9 +5,10 @@
 // thread's 16 registers (the "register bits", 4 of them) is the *layout*;
 // the remaining B-4 tile qubits index the thread.  Gates whose matrix qubits
 // are register bits are applied entirely in registers; a layout change is one
-// LDS round trip (write all, barrier, read all) with an XOR swizzle so that
-// strided layouts stay free of bank conflicts.  The host plans, per pass, the
-// layout every op runs in, so most gates cost no LDS traffic and no barrier.
+// LDS round trip (write all, barrier, read all) with an XOR swizzle that
+// spreads strided layouts over the banks (the per-register offsets come from a
+// host table).  The host plans, per pass, the layout every op runs in, so most
+// gates cost no LDS traffic and no barrier.
 //
 // Arithmetic: each output is the reference's row sum (src/engine.rs:58-59,
 // 146-151) in the same column order.  The host picks a variant per op:
