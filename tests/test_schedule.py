@@ -175,3 +175,18 @@ def test_layout_table_matches_register_bits(b, elem):
         for j1 in range(16):
             for j2 in range(16):
                 assert table[j1 ^ j2] == table[j1] ^ table[j2]
+
+
+def test_tile_choice_cost_at_flush_size():
+    """The engine flushes at 4096 queued ops; planning them for a 30-qubit
+    state stays far below the time of one sweep (about 10 ms)."""
+    import time
+
+    rng = np.random.default_rng(5)
+    ops = layered_ops(30, 90, rng)[:4096]
+    args = [(qs, np.asarray(u).ravel(), cv) for qs, u, cv in ops]
+    t0 = time.perf_counter()
+    passes = core._schedule(30, args, 2, 12, 4)
+    elapsed = time.perf_counter() - t0
+    assert sorted(i for _, idx in passes for i in idx) == list(range(len(ops)))
+    assert elapsed < 0.5, elapsed
