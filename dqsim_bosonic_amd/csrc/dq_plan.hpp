@@ -639,6 +639,105 @@ struct RegPlan {
     int fused_gates = 0;   // gates they replace
 };
 
+// Register-layout order of a pass's ops (fusion level 2 and up).  A layout
+// holds 4 tile bits; every change of layout is an LDS round trip of the whole
+// tile with two barriers.  Ops commute when their tile bits are disjoint or
+// both are diagonal (ops touch outer qubits only diagonally), so they may be
+// reordered within those limits: repeatedly run every ready op whose register
+// bits fit the current layout, then pick the next layout as the one (built
+// from a ready op and the needs of the ops after it, as the layout planner
+// builds them) that lets the most ops run.  Passes of more than kOrderMaxOps
+// ops keep program order (the search is quadratic in the op count).
+constexpr size_t kOrderMaxOps = 64;
+
+inline std::vector<const HostOp *> order_for_layouts(const std::vector<const HostOp *> &in, const int *pos) {
+    const size_t n = in.size();
+    if (n < 3 || n > kOrderMaxOps) return in;
+    std::vector<unsigned> need(n, 0), soft(n, 0), qm(n, 0);
+    std::vector<char> diag(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+        const HostOp &h = *in[i];
+        bool all_inner = true;
+        for (int j = 0; j < h.k; ++j) {
+            if (pos[h.q[j]] < kOuterBit) qm[i] |= 1u << pos[h.q[j]];
+            else all_inner = false;
+        }
+        diag[i] = (h.kind == K_DIAG || h.kind == K_COLLAPSE) ? 1 : 0;
+        if (h.kind == K_DENSE1 || h.kind == K_MONO || h.kind == K_DENSEK) {
+            if (h.k <= 2) need[i] = qm[i];
+        } else if (h.kind == K_CTRL1) {
+            need[i] = 1u << pos[h.q[h.k - 1]];
+        } else if (h.kind == K_DIAG && h.k <= 2 && all_inner) {
+            soft[i] = qm[i];
+        }
+    }
+    // i must precede j: shared tile bits, not both diagonal
+    std::vector<std::vector<int>> succ(n);
+    std::vector<int> indeg(n, 0);
+    for (size_t j = 0; j < n; ++j)
+        for (size_t i = 0; i < j; ++i)
+            if ((qm[i] & qm[j]) && !(diag[i] && diag[j])) {
+                succ[i].push_back(int(j));
+                ++indeg[j];
+            }
+    std::vector<char> done(n, 0);
+    std::vector<int> deg = indeg;
+    // runs every op that becomes ready and fits layout S, in program order;
+    // returns how many ran (the state is updated in place)
+    auto run = [&](unsigned S, std::vector<char> &dn, std::vector<int> &dg, std::vector<int> *order) {
+        int cnt = 0;
+        for (bool progress = true; progress;) {
+            progress = false;
+            for (size_t k = 0; k < n; ++k) {
+                if (dn[k] || dg[k] || (need[k] & ~S)) continue;
+                dn[k] = 1;
+                for (int j : succ[k]) --dg[j];
+                if (order) order->push_back(int(k));
+                ++cnt;
+                progress = true;
+            }
+        }
+        return cnt;
+    };
+    std::vector<int> order;
+    run(0u, done, deg, &order);
+    std::vector<unsigned> cands;
+    while (order.size() < n) {
+        cands.clear();
+        for (size_t k = 0; k < n; ++k) {
+            if (done[k] || deg[k]) continue;
+            for (int mode = 0; mode < 2; ++mode) {
+                unsigned S = need[k];
+                for (size_t j = 0; j < n; ++j) {
+                    if (done[j] || j == k) continue;
+                    const unsigned nd = need[j] ? need[j] : (mode ? soft[j] : 0u);
+                    if (!nd || (nd & ~S) == 0) continue;
+                    if (__builtin_popcount(S | nd) <= 4) S |= nd;
+                    else if (need[j] && mode == 0) break;
+                }
+                if (std::find(cands.begin(), cands.end(), S) == cands.end()) cands.push_back(S);
+            }
+        }
+        int best = -1;
+        unsigned bestS = 0;
+        for (unsigned S : cands) {
+            std::vector<char> dn = done;
+            std::vector<int> dg = deg;
+            const int c = run(S, dn, dg, nullptr);
+            if (c > best) {
+                best = c;
+                bestS = S;
+            }
+        }
+        if (best <= 0) break;  // cannot happen: a ready op always fits a layout built from it
+        run(bestS, done, deg, &order);
+    }
+    if (order.size() != n) return in;
+    std::vector<const HostOp *> out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = in[size_t(order[i])];
+    return out;
+}
+
 // The kernel's LDS swizzle of a tile index (dq_tile.hpp: swz): the low nibble
 // XORed with every higher nibble of the b tile bits.
 inline unsigned tile_swizzle(unsigned i, int b) {
@@ -693,6 +792,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
             out.fused_gates += f.outcome;
         }
     }
+    if (s->fuse >= 2) eff = order_for_layouts(eff, pos);
     const size_t nops = eff.size();
     std::vector<unsigned> hard(nops, 0), soft(nops, 0);
     for (size_t i = 0; i < nops; ++i) {

@@ -397,6 +397,20 @@ PYBIND11_MODULE(_core, m) {
         return out;
     }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
        py::arg("reg_tiles") = true);
+    // The register-layout order of one pass's ops (dq_plan.hpp:
+    // order_for_layouts); qubits not in tile_qubits are outer.  Returns the
+    // op indices in the order the pass would apply them.
+    m.def("_order_for_layouts", [](std::vector<int> tile_qubits, py::list ops) {
+        std::vector<dq::HostOp> queue = host_ops(ops);
+        int pos[64];
+        for (int q = 0; q < 64; ++q) pos[q] = dq::kOuterBit + q;
+        for (size_t t = 0; t < tile_qubits.size() && t < 16; ++t) pos[tile_qubits[t]] = int(t);
+        std::vector<const dq::HostOp *> in;
+        for (const dq::HostOp &h : queue) in.push_back(&h);
+        std::vector<int> out;
+        for (const dq::HostOp *h : dq::order_for_layouts(in, pos)) out.push_back(int(h - queue.data()));
+        return out;
+    }, py::arg("tile_qubits"), py::arg("ops"));
     // One layout-table entry of the register sweep (dq_plan.hpp): the 16 LDS
     // byte offsets of layout rb for tiles of b qubits and amplitudes of elem bytes.
     m.def("_layout_table", [](int rb, int b, unsigned elem) {

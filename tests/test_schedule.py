@@ -190,3 +190,72 @@ def test_tile_choice_cost_at_flush_size():
     elapsed = time.perf_counter() - t0
     assert sorted(i for _, idx in passes for i in idx) == list(range(len(ops)))
     assert elapsed < 0.5, elapsed
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("n,tile", [(9, list(range(9))), (12, [0, 1, 2, 3, 5, 6, 8, 9, 10, 11])])
+def test_layout_order_equals_program(seed, n, tile):
+    """Ops reordered inside a pass so that layouts serve more ops: only
+    commuting ops (disjoint tile qubits, or both diagonal; outer qubits are
+    touched diagonally) change places, so the state equals the program's."""
+    rng = np.random.default_rng(300 + seed)
+    ops = []
+    for qs, u, cv in random_ops(n, 50, rng):
+        d = np.asarray(u).reshape(int(np.sqrt(np.asarray(u).size)), -1)
+        diag_only = is_diag(u, cv)
+        outer = [q for q in qs if q not in tile]
+        # in a tile pass, ops touch outer qubits only diagonally (or as controls)
+        if outer and not diag_only and not (cv and qs[-1] in tile):
+            continue
+        ops.append((qs, u, cv))
+    for _ in range(12):
+        qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+        ops.insert(int(rng.integers(len(ops) + 1)), (qs, orc.gate_matrix("cp", rng.uniform(-4, 4, 1)), []))
+    order = core._order_for_layouts(tile, [(qs, np.asarray(u).ravel(), cv) for qs, u, cv in ops])
+    assert sorted(order) == list(range(len(ops)))
+    psi0 = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    want = psi0.copy()
+    for qs, u, cv in ops:
+        want = apply_op(want, n, qs, u, cv)
+    got = psi0.copy()
+    for i in order:
+        got = apply_op(got, n, *ops[i])
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+def test_layout_order_groups_register_needs():
+    """Two interleaved streams of gates on disjoint pairs need a relayout
+    per gate in program order; reordered, each pair's gates run together."""
+    n = 12
+    tile = list(range(n))
+    h = orc.gate_matrix("h", [])
+    cx = orc.gate_matrix("cx", [])
+    ops = []
+    for _ in range(4):
+        for a, b in ((4, 5), (6, 7), (8, 9)):
+            ops.append(([a], h, []))
+            ops.append(([a, b], cx, []))
+    order = core._order_for_layouts(tile, [(qs, np.asarray(u).ravel(), cv) for qs, u, cv in ops])
+
+    def relayouts(seq):
+        """Layout changes a 4-bit register layout needs for this op order,
+        each new layout covering the longest run of ops ahead (the planner's
+        rule)."""
+        seq = list(seq)
+        cur, count = set(), 0
+        for i, k in enumerate(seq):
+            need = set(ops[k][0])
+            if need <= cur:
+                continue
+            cur = set(need)
+            for k2 in seq[i + 1:]:
+                if set(ops[k2][0]) <= cur:
+                    continue
+                if len(cur | set(ops[k2][0])) > 4:
+                    break
+                cur |= set(ops[k2][0])
+            count += 1
+        return count
+
+    assert relayouts(order) < relayouts(range(len(ops)))
