@@ -421,10 +421,9 @@ __device__ __forceinline__ void fused5_mfma(cplx<float> *lds, const cplx<float> 
 // also carries the MFMA block op (T_FUSED5), kept apart because its
 // accumulators cost registers (occupancy) in sweeps that have no blocks.
 template <typename T, int B, int FAST>
-__global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__ amps, TilePass pd,
-                                                           const TileOp *__restrict__ ops,
-                                                           const cplx<T> *__restrict__ mats, uint64_t ntiles,
-                                                           double *__restrict__ partials) {
+__device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const TilePass &pd,
+                                           const TileOp *__restrict__ ops, const cplx<T> *__restrict__ mats,
+                                           uint64_t ntiles, double *__restrict__ partials) {
     double red = 0.0;  // this thread's share of the epilogue reduction
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -688,6 +687,25 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
             partials[blockIdx.x] = r;
         }
     }
+}
+
+template <typename T, int B, int FAST>
+__global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__ amps, TilePass pd,
+                                                           const TileOp *__restrict__ ops,
+                                                           const cplx<T> *__restrict__ mats, uint64_t ntiles,
+                                                           double *__restrict__ partials) {
+    tile_sweep<T, B, FAST>(amps, pd, ops, mats, ntiles, partials);
+}
+
+// complex64 without the MFMA block op: the sweep fits 128 VGPRs, and a
+// 12-qubit tile takes 32 KiB of LDS, so 4 waves per SIMD (one more than the
+// register allocation reaches unasked)
+template <int B>
+__global__ __launch_bounds__(1 << (B - 4), 4) void k_tile_reg_c64(cplx<float> *__restrict__ amps, TilePass pd,
+                                                                  const TileOp *__restrict__ ops,
+                                                                  const cplx<float> *__restrict__ mats,
+                                                                  uint64_t ntiles, double *__restrict__ partials) {
+    tile_sweep<float, B, 1>(amps, pd, ops, mats, ntiles, partials);
 }
 
 }  // namespace dq
