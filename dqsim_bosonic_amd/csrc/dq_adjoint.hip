@@ -410,7 +410,7 @@ __device__ __forceinline__ double r_grad_any(const cplx<T> (&p)[NR], const cplx<
 }
 
 template <typename T, int B, int RB>
-__global__ __launch_bounds__(1 << (B - RB), 2) void k_adjoint_reg(cplx<T> *__restrict__ psi, cplx<T> *__restrict__ lam,
+__global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoint_reg(cplx<T> *__restrict__ psi, cplx<T> *__restrict__ lam,
                                                               AdjPass pd, const AdjRStep *__restrict__ steps,
                                                               const cplx<double> *__restrict__ mats,
                                                               const AdjRTerm *__restrict__ terms,
