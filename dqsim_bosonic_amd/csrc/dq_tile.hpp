@@ -697,15 +697,16 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
     tile_sweep<T, B, FAST>(amps, pd, ops, mats, ntiles, partials);
 }
 
-// complex64 without the MFMA block op: the sweep fits 128 VGPRs, and a
-// 12-qubit tile takes 32 KiB of LDS, so 4 waves per SIMD (one more than the
-// register allocation reaches unasked)
-template <int B>
+// complex64 without the MFMA block op (FAST = 1): the sweep fits 128 VGPRs,
+// and a 12-qubit tile takes 32 KiB of LDS, so 4 waves per SIMD (one more than
+// the register allocation reaches unasked).  The MFMA instantiation would
+// spill at this bound and keeps k_tile_reg.
+template <int B, int FAST>
 __global__ __launch_bounds__(1 << (B - 4), 4) void k_tile_reg_c64(cplx<float> *__restrict__ amps, TilePass pd,
                                                                   const TileOp *__restrict__ ops,
                                                                   const cplx<float> *__restrict__ mats,
                                                                   uint64_t ntiles, double *__restrict__ partials) {
-    tile_sweep<float, B, 1>(amps, pd, ops, mats, ntiles, partials);
+    tile_sweep<float, B, FAST>(amps, pd, ops, mats, ntiles, partials);
 }
 
 }  // namespace dq

@@ -12,11 +12,11 @@ hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass 
     using T = float;
     const size_t shmem = sizeof(cplx<T>) << B;
     if (shmem > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg_c64<B>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg_c64<B, 1>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_tile_reg_c64<B>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
+    hipLaunchKernelGGL((k_tile_reg_c64<B, 1>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
                        static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles, partials);
     return hipGetLastError();
 }
