@@ -946,25 +946,74 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
     const int b = std::min(n, 11);
     const uint64_t low = (uint64_t(1) << kLowBits) - 1;
     // passes: ops in reverse order; an op joins the pass if its qubits fit
-    // the tile and miss every qubit of an op left for a later pass
+    // the tile and miss every qubit of an op left for a later pass.  The
+    // tile's qubits are chosen first: starting from the low block, add the
+    // qubits of the waiting op whose addition lets the pass take the most
+    // ops over the next 256 (ties: fewer qubits, then the earlier op).
     std::vector<std::vector<int>> passes;
     std::vector<uint64_t> pass_S;
     {
         std::vector<int> remaining;
         for (int l = n_ops - 1; l >= 0; --l) remaining.push_back(l);
-        while (!remaining.empty()) {
-            uint64_t S = low, blocked = 0;
-            std::vector<int> take, rest;
-            for (int l : remaining) {
-                const uint64_t q = qmask[size_t(l)];
-                if (!(q & blocked) && __builtin_popcountll(S | q) <= b) {
-                    S |= q;
-                    take.push_back(l);
-                } else {
+        auto fitting = [&](uint64_t S, std::vector<uint64_t> *grow) {
+            uint64_t blocked = 0;
+            int taken = 0;
+            const size_t end = std::min<size_t>(remaining.size(), 256);
+            for (size_t r = 0; r < end; ++r) {
+                const uint64_t q = qmask[size_t(remaining[r])];
+                if (q & blocked) {  // held back behind a deferred op, as the pass builder does
                     blocked |= q;
-                    rest.push_back(l);
+                    continue;
                 }
+                if ((q & ~S) == 0) {
+                    ++taken;
+                    continue;
+                }
+                blocked |= q;
+                if (grow && std::find(grow->begin(), grow->end(), q & ~S) == grow->end()) grow->push_back(q & ~S);
             }
+            return taken;
+        };
+        std::vector<uint64_t> grow;
+        while (!remaining.empty()) {
+            uint64_t S = low;
+            for (;;) {
+                grow.clear();
+                const int base = fitting(S, &grow);
+                int best_gain = 0, best_bits = 0;
+                uint64_t best = 0;
+                for (uint64_t extra : grow) {
+                    const int bits = __builtin_popcountll(extra);
+                    if (__builtin_popcountll(S | extra) > b) continue;
+                    const int gain = fitting(S | extra, nullptr) - base;
+                    if (gain > best_gain || (gain == best_gain && gain > 0 && bits < best_bits)) {
+                        best_gain = gain;
+                        best_bits = bits;
+                        best = extra;
+                    }
+                }
+                if (!best) break;
+                S |= best;
+            }
+            std::vector<int> take, rest;
+            auto build_pass = [&](uint64_t S0) {
+                take.clear();
+                rest.clear();
+                S = S0;
+                uint64_t blocked = 0;
+                for (int l : remaining) {
+                    const uint64_t q = qmask[size_t(l)];
+                    if (!(q & blocked) && __builtin_popcountll(S | q) <= b) {
+                        S |= q;
+                        take.push_back(l);
+                    } else {
+                        blocked |= q;
+                        rest.push_back(l);
+                    }
+                }
+            };
+            build_pass(S);
+            if (take.empty()) build_pass(low);  // the chosen tile admits an op by construction; kept as a guard
             if (take.empty()) return afail(DQ_ERR_ARG, "a gate does not fit the tile");
             // fill the tile with the lowest unused qubits
             for (int q = 0; q < n && __builtin_popcountll(S) < b; ++q) S |= uint64_t(1) << q;
