@@ -933,6 +933,7 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
             qmask[size_t(l)] |= uint64_t(1) << q;
         }
         if (n_terms[l] < 0 || n_terms[l] > 64) return afail(DQ_ERR_ARG, "generator term count must be 0..64");
+        if (n_terms[l] > 0 && (!xmask || !zmask || !coeff)) return afail(DQ_ERR_ARG, "null argument");
         mat_off[size_t(l) + 1] = mat_off[size_t(l)] + (size_t(1) << (2 * k[l]));
         term_off[size_t(l) + 1] = term_off[size_t(l)] + size_t(n_terms[l]);
         for (size_t t = term_off[size_t(l)]; t < term_off[size_t(l) + 1]; ++t)
@@ -1115,48 +1116,67 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
         hpasses.push_back(pd);
         pass_grad_ops.push_back(std::move(gops));
     }
-    void *d_mats = nullptr, *d_steps = nullptr, *d_terms = nullptr, *d_rsteps = nullptr, *d_rterms = nullptr;
-    double *d_part = nullptr, *d_grads = nullptr;
-    auto release = [&]() {
-        if (d_rsteps) (void)hipFree(d_rsteps);
-        if (d_rterms) (void)hipFree(d_rterms);
-        if (d_mats) (void)hipFree(d_mats);
-        if (d_steps) (void)hipFree(d_steps);
-        if (d_terms) (void)hipFree(d_terms);
-        if (d_part) (void)hipFree(d_part);
-        if (d_grads) (void)hipFree(d_grads);
-    };
     const size_t slots_total = std::max<size_t>(1, [&] {
         size_t s = 0;
         for (const auto &g : pass_grad_ops) s += g.size();
         return s;
     }());
-    if (hipMalloc(&d_mats, std::max<size_t>(1, hmats.size()) * sizeof(cplx<double>)) != hipSuccess ||
-        hipMalloc(&d_steps, std::max<size_t>(1, hsteps.size()) * sizeof(AdjStep)) != hipSuccess ||
-        hipMalloc(&d_terms, std::max<size_t>(1, hterms.size()) * sizeof(AdjTerm)) != hipSuccess ||
-        hipMalloc(&d_rsteps, std::max<size_t>(1, rsteps.size()) * sizeof(AdjRStep)) != hipSuccess ||
-        hipMalloc(&d_rterms, std::max<size_t>(1, rterms.size()) * sizeof(AdjRTerm)) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void **>(&d_part), max_slots * ntiles * 4 * sizeof(double)) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void **>(&d_grads), slots_total * sizeof(double)) != hipSuccess) {
-        release();
-        return afail(DQ_ERR_OOM, "hipMalloc of the sweep's buffers failed");
+    // One device block (grow-only, kept on psi across calls) holds the tables
+    // and the partials; the tables are staged in one pinned host block and
+    // uploaded with one copy.  Every region starts on a 256-byte boundary.
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t b_mats = up(std::max<size_t>(1, hmats.size()) * sizeof(cplx<double>));
+    const size_t b_steps = up(std::max<size_t>(1, hsteps.size()) * sizeof(AdjStep));
+    const size_t b_terms = up(std::max<size_t>(1, hterms.size()) * sizeof(AdjTerm));
+    const size_t b_rsteps = up(std::max<size_t>(1, rsteps.size()) * sizeof(AdjRStep));
+    const size_t b_rterms = up(std::max<size_t>(1, rterms.size()) * sizeof(AdjRTerm));
+    const size_t b_tables = b_mats + b_steps + b_terms + b_rsteps + b_rterms;
+    const size_t b_part = up(max_slots * ntiles * 4 * sizeof(double));
+    const size_t b_grads = up(slots_total * sizeof(double));
+    const size_t b_total = b_tables + b_part + b_grads;
+    if (psi->d_adj_cap < b_total) {
+        if (psi->d_adj) (void)hipFree(psi->d_adj);
+        psi->d_adj = nullptr;
+        psi->d_adj_cap = 0;
+        if (hipMalloc(&psi->d_adj, b_total) != hipSuccess) {
+            psi->d_adj = nullptr;
+            return afail(DQ_ERR_OOM, "hipMalloc of the sweep's buffers failed");
+        }
+        psi->d_adj_cap = b_total;
+    }
+    if (psi->h_adj_cap < b_tables + b_grads) {
+        if (psi->h_adj) (void)hipHostFree(psi->h_adj);
+        psi->h_adj = nullptr;
+        psi->h_adj_cap = 0;
+        if (hipHostMalloc(reinterpret_cast<void **>(&psi->h_adj), b_tables + b_grads, hipHostMallocDefault) !=
+            hipSuccess) {
+            psi->h_adj = nullptr;
+            return afail(DQ_ERR_OOM, "hipHostMalloc of the sweep's staging buffer failed");
+        }
+        psi->h_adj_cap = b_tables + b_grads;
+    }
+    unsigned char *dbase = static_cast<unsigned char *>(psi->d_adj);
+    void *d_mats = dbase;
+    void *d_steps = dbase + b_mats;
+    void *d_terms = dbase + b_mats + b_steps;
+    void *d_rsteps = dbase + b_mats + b_steps + b_terms;
+    void *d_rterms = dbase + b_mats + b_steps + b_terms + b_rsteps;
+    double *d_part = reinterpret_cast<double *>(dbase + b_tables);
+    double *d_grads = reinterpret_cast<double *>(dbase + b_tables + b_part);
+    {
+        unsigned char *h = psi->h_adj;
+        std::memcpy(h, hmats.data(), hmats.size() * sizeof(cplx<double>));
+        std::memcpy(h + b_mats, hsteps.data(), hsteps.size() * sizeof(AdjStep));
+        if (!hterms.empty()) std::memcpy(h + b_mats + b_steps, hterms.data(), hterms.size() * sizeof(AdjTerm));
+        if (!rsteps.empty())
+            std::memcpy(h + b_mats + b_steps + b_terms, rsteps.data(), rsteps.size() * sizeof(AdjRStep));
+        if (!rterms.empty())
+            std::memcpy(h + b_mats + b_steps + b_terms + b_rsteps, rterms.data(), rterms.size() * sizeof(AdjRTerm));
     }
     hipStream_t stream = psi->stream;
     int rc = DQ_OK;
     auto run = [&]() -> int {
-        ADJ_TRY(hipMemcpyAsync(d_mats, hmats.data(), hmats.size() * sizeof(cplx<double>), hipMemcpyHostToDevice,
-                               stream));
-        ADJ_TRY(hipMemcpyAsync(d_steps, hsteps.data(), hsteps.size() * sizeof(AdjStep), hipMemcpyHostToDevice,
-                               stream));
-        if (!hterms.empty())
-            ADJ_TRY(hipMemcpyAsync(d_terms, hterms.data(), hterms.size() * sizeof(AdjTerm), hipMemcpyHostToDevice,
-                                   stream));
-        if (!rsteps.empty())
-            ADJ_TRY(hipMemcpyAsync(d_rsteps, rsteps.data(), rsteps.size() * sizeof(AdjRStep), hipMemcpyHostToDevice,
-                                   stream));
-        if (!rterms.empty())
-            ADJ_TRY(hipMemcpyAsync(d_rterms, rterms.data(), rterms.size() * sizeof(AdjRTerm), hipMemcpyHostToDevice,
-                                   stream));
+        ADJ_TRY(hipMemcpyAsync(d_mats, psi->h_adj, b_tables, hipMemcpyHostToDevice, stream));
         // a power of two: the register kernel derives tile offsets from it
         const unsigned grid = unsigned(std::min<uint64_t>(ntiles, 8192));
         if (grid & (grid - 1)) return afail(DQ_ERR_ARG, "grid is not a power of two");
@@ -1193,8 +1213,8 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
             psi->launches++;
             psi->sweeps++;
         }
-        std::vector<double> hg(slots_total);
-        ADJ_TRY(hipMemcpyAsync(hg.data(), d_grads, slots_total * sizeof(double), hipMemcpyDeviceToHost, stream));
+        double *hg = reinterpret_cast<double *>(psi->h_adj + b_tables);
+        ADJ_TRY(hipMemcpyAsync(hg, d_grads, slots_total * sizeof(double), hipMemcpyDeviceToHost, stream));
         ADJ_TRY(hipStreamSynchronize(stream));
         size_t s = 0;
         for (const auto &g : pass_grad_ops)
@@ -1203,7 +1223,6 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
     };
     rc = run();
     (void)hipStreamSynchronize(stream);
-    release();
     return rc;
 }
 

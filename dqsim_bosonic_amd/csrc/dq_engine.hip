@@ -950,6 +950,8 @@ void dq_state_destroy(dq_state *s) {
     if (s->d_partials) (void)hipFree(s->d_partials);
     if (s->d_result) (void)hipFree(s->d_result);
     if (s->h_result) (void)hipHostFree(s->h_result);
+    if (s->d_adj) (void)hipFree(s->d_adj);
+    if (s->h_adj) (void)hipHostFree(s->h_adj);
     if (s->upload_done) (void)hipEventDestroy(s->upload_done);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;

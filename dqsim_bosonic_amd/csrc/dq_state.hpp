@@ -28,6 +28,12 @@ struct dq_state {
     unsigned char *h_upload = nullptr;
     size_t h_upload_cap = 0;
     hipEvent_t upload_done = nullptr;
+    // adjoint sweep buffers (grow-only, reused across calls): device tables and
+    // partials, and a pinned staging copy of the tables for one upload
+    void *d_adj = nullptr;
+    size_t d_adj_cap = 0;
+    unsigned char *h_adj = nullptr;
+    size_t h_adj_cap = 0;
     // reduction buffers
     double *d_partials = nullptr;
     double *h_result = nullptr;  // pinned, 64 doubles

@@ -154,6 +154,14 @@ PYBIND11_MODULE(_core, m) {
         .def_readonly("measure_qubit_time", &SimulationProfile::mq_time)
         .def_readonly("total_time", &SimulationProfile::total_time)
         .def("__repr__", &SimulationProfile::repr);
+    // A profile with given fields, for checking the __repr__ format on fixed values.
+    m.def("_profile_repr", [](uint64_t oq, double oqt, uint64_t nq, double nqt, uint64_t mq, double mqt,
+                              double total) {
+        SimulationProfile p;
+        p.oq_calls = oq, p.oq_time = oqt, p.nq_calls = nq, p.nq_time = nqt, p.mq_calls = mq, p.mq_time = mqt;
+        p.total_time = total;
+        return p.repr();
+    });
 
     py::class_<SimulationResult>(m, "SimulationResult")
         .def_property_readonly("statevector", &SimulationResult::statevector)
@@ -272,6 +280,7 @@ PYBIND11_MODULE(_core, m) {
         .def("set_fuse_min_cost", &Engine::set_fuse_min_cost)
         .def("set_pair_fusion", &Engine::set_pair_fusion)
         .def("compare", &Engine::compare)
+        .def("p1_record", &Engine::p1_record)
         .def("result", &Engine::result);
 
     m.def("simulate_batch", &simulate_batch, py::arg("circuits"), py::arg("observable") = py::none(),

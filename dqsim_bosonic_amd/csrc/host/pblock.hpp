@@ -244,10 +244,12 @@ class PBlockResult {
     py::array_t<std::complex<double>> statevector() const {
         py::array_t<std::complex<double>> out(static_cast<py::ssize_t>(size_t(1) << blk_.qubits.size()));
         auto *p = reinterpret_cast<double *>(out.mutable_data());
+        int rc;
         {
             py::gil_scoped_release nogil;
-            check(dq_copy_to_host(blk_.st->s, p), "statevector copy");
+            rc = dq_copy_to_host(blk_.st->s, p);
         }
+        check(rc, "statevector copy");
         return out;
     }
     size_t num_qubits() const { return blk_.qubits.size(); }

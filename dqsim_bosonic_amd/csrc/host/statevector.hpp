@@ -166,6 +166,7 @@ class Engine {
         Runner run(st_->s, n_, seed, false);
         for (const Instr &in : circ_.instructions) run.run(in);
         check(dq_synchronize(st_->s), "synchronize");
+        p1s_ = run.p1s;
         py::dict d;
         for (const auto &kv : run.cbits) d[py::int_(kv.first)] = py::int_(kv.second);
         return d;
@@ -202,11 +203,14 @@ class Engine {
         return d;
     }
     SimulationResult result() const { return SimulationResult(st_, n_, {}, py::none()); }
+    // p1 of each measurement and reset of the last step, in program order
+    const std::vector<double> &p1_record() const { return p1s_; }
 
    private:
     int n_;
     std::shared_ptr<DeviceState> st_;
     Circuit circ_;
+    std::vector<double> p1s_;
 };
 
 }  // namespace

@@ -51,6 +51,7 @@ class Runner {
 
     std::map<uint64_t, int> cbits;
     SimulationProfile acc;
+    std::vector<double> p1s;  // the p1 of every measurement and reset, in program order
 
     // run_instruction, simulator.rs:386-632
     void run(const Instr &in) {
@@ -138,6 +139,7 @@ class Runner {
         timed(acc.mq_calls, acc.mq_time, [&] {
             double p1 = 0.0;
             check(dq_prob_one(s_, q, &p1), "prob_one");
+            p1s.push_back(p1);
             outcome = rng_.gen_f64() < p1 ? 1 : 0;
             const double norm =
                 outcome ? std::max(std::sqrt(p1), 1e-15) : std::max(std::sqrt(std::max(1.0 - p1, 0.0)), 1e-15);

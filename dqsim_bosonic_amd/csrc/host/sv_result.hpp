@@ -77,6 +77,10 @@ struct SimulationProfile {
 
 using Clock = std::chrono::steady_clock;
 
+// The qubit list of probabilities() / counts(): the default is [n-1, ..., 0]
+// (simulator.rs:150-152).  As in the reference, which takes Vec<usize>
+// (simulator.rs:148), a negative index is an OverflowError; indices at or
+// above n are accepted (see SimulationResult::marginal).
 std::vector<int> qubits_list(const std::optional<std::vector<int64_t>> &qubits, int n) {
     std::vector<int> qs;
     if (!qubits) {
@@ -84,7 +88,12 @@ std::vector<int> qubits_list(const std::optional<std::vector<int64_t>> &qubits, 
         return qs;
     }
     for (int64_t q : *qubits) {
-        if (q < 0 || q >= n) throw py::value_error("qubit index " + std::to_string(q) + " out of range");
+        if (q < 0) {
+            PyErr_SetString(PyExc_OverflowError, "can't convert negative int to unsigned");
+            throw py::error_already_set();
+        }
+        // the reference's mask (1 << q) needs q < 64
+        if (q >= 64) throw py::value_error("qubit index " + std::to_string(q) + " out of range");
         qs.push_back(int(q));
     }
     return qs;
@@ -101,10 +110,12 @@ class SimulationResult {
     py::array_t<std::complex<double>> statevector() const {
         py::array_t<std::complex<double>> out(size_t(1) << n_);
         auto *p = reinterpret_cast<double *>(out.mutable_data());
+        int rc;
         {
             py::gil_scoped_release nogil;
-            check(dq_copy_to_host(st_->s, p), "statevector copy");
+            rc = dq_copy_to_host(st_->s, p);
         }
+        check(rc, "statevector copy");
         return out;
     }
 
@@ -116,10 +127,57 @@ class SimulationResult {
 
     py::object profile() const { return prof_; }
 
+    // engine::marginal_probs (src/engine.rs:368-390): probs[j] sums |psi_i|^2
+    // over the i with (i & mask) == desired_j, mask and desired_j built from
+    // the listed qubits (qubits[0] = MSB of j).  The device computes the
+    // marginal over the distinct listed qubits below n; each bin j then takes
+    // the bin of its desired pattern, which is the same set of terms.  A
+    // qubit at or above n makes every bin that wants it set empty (0.0); a
+    // repeated qubit makes every j that sets it in any position want it set.
     std::vector<double> marginal(const std::vector<int> &qs) const {
+        if (qs.size() > 34) throw py::value_error("at most 34 qubits");
         std::vector<double> probs(size_t(1) << qs.size());
-        check(dq_marginal_probs(st_->s, qs.data(), int(qs.size()), probs.data()), "marginal probabilities");
+        marginal_to(qs, probs.data());
         return probs;
+    }
+    void marginal_to(const std::vector<int> &qs, double *out) const {
+        const int k = int(qs.size());
+        std::vector<int> distinct;
+        bool plain = true;
+        uint64_t seen = 0;
+        for (int q : qs) {
+            if (q >= n_ || ((seen >> q) & 1)) {
+                plain = false;
+                continue;
+            }
+            seen |= uint64_t(1) << q;
+            distinct.push_back(q);
+        }
+        std::vector<double> sub(plain ? 0 : size_t(1) << distinct.size());
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = dq_marginal_probs(st_->s, distinct.data(), int(distinct.size()), plain ? out : sub.data());
+        }
+        check(rc, "marginal probabilities");
+        if (plain) return;
+        const int kd = int(distinct.size());
+        std::vector<int> pos(64, -1);  // qubit -> bit of the distinct marginal's index (MSB first)
+        for (int j = 0; j < kd; ++j) pos[size_t(distinct[size_t(j)])] = kd - 1 - j;
+        for (size_t j = 0; j < (size_t(1) << k); ++j) {
+            bool empty = false;
+            size_t jd = 0;
+            for (int b = 0; b < k; ++b) {
+                if (!((j >> (k - 1 - b)) & 1)) continue;
+                const int q = qs[size_t(b)];
+                if (q >= n_) {
+                    empty = true;
+                    break;
+                }
+                jd |= size_t(1) << pos[size_t(q)];
+            }
+            out[j] = empty ? 0.0 : sub[jd];
+        }
     }
 
     // Lazy results (no reference counterpart): the distribution as an array
@@ -128,11 +186,7 @@ class SimulationResult {
         const std::vector<int> qs = qubits_list(qubits, n_);
         if (qs.size() > 34) throw py::value_error("at most 34 qubits");
         py::array_t<double> out(static_cast<py::ssize_t>(size_t(1) << qs.size()));
-        double *p = out.mutable_data();
-        {
-            py::gil_scoped_release nogil;
-            check(dq_marginal_probs(st_->s, qs.data(), int(qs.size()), p), "marginal probabilities");
-        }
+        marginal_to(qs, out.mutable_data());
         return out;
     }
 

@@ -253,6 +253,9 @@ class OracleSimulator:
     def __init__(self, seed: int, parallel: bool = False):
         self.seed = seed
         self.parallel = parallel
+        # engine calls by the reference's profiling wrappers (simulator.rs:890-946):
+        # do_oq / do_nq / do_mq, one count per call that the dispatcher makes
+        self.calls = {"apply_one_qubit": 0, "apply_n_qubit": 0, "measure_qubit": 0}
 
     def simulate_json(self, text: str, max_instructions: int | None = None) -> OracleResult:
         circ = json.loads(text)
@@ -326,6 +329,7 @@ class OracleSimulator:
         self._run(state, n, inst, cbits, rng, [])
 
     def _oq(self, state, n, u, q):
+        self.calls["apply_one_qubit"] += 1
         if self.parallel and n >= 12:
             u = np.ascontiguousarray(u, dtype=np.complex128)
             lib().or_apply_one_qubit_par(_dp(state.view(np.float64)), n, _dp(u.view(np.float64)), q)
@@ -333,9 +337,11 @@ class OracleSimulator:
             apply_one_qubit(state, n, u, q)
 
     def _nq(self, state, n, u, qs):
+        self.calls["apply_n_qubit"] += 1
         apply_n_qubit(state, n, u, qs, parallel=self.parallel and n >= 12)
 
     def _measure(self, state, n, q, rng, p1s):
+        self.calls["measure_qubit"] += 1
         d = _dp(state.view(np.float64))
         if self.parallel and n >= 12:
             p1 = float(lib().or_prob_one_par(d, n, q))

@@ -9,6 +9,7 @@ Known answers restated from the reference's files:
 """
 from __future__ import annotations
 
+import json
 import math
 
 import numpy as np
@@ -123,15 +124,48 @@ def test_fidelity_and_length_mismatch():
     assert r.fidelity(np.zeros(8, dtype=np.complex128)) == 0.0
 
 
-def test_profile_counts_calls():
-    circ = W.random_blocks(10, 10, seed=5, remote_per_kind=2, measure_every=5)
-    r = dq.StatevectorSimulator(seed=5, profile=True).simulate(circ)
-    p = r.profile
-    n1 = sum(1 for i in circ.instructions if i.kind in ("h", "s", "t", "sx", "rx", "ry", "rz"))
-    assert p.measure_qubit_calls == 8  # 2 measures + 2 resets per checkpoint, 2 checkpoints
-    assert p.apply_one_qubit_calls >= n1
-    assert p.total_time > 0
-    assert "SimulationProfile (total:" in repr(p)
+class _JsonCircuit:
+    def __init__(self, d):
+        self._text = json.dumps(d)
+
+    def model_dump_json(self):
+        return self._text
+
+
+def _profile_circuits():
+    from tests.test_oracle import profile_case
+
+    yield "known", _JsonCircuit(profile_case()[0])
+    yield "blocks", W.random_blocks(10, 10, seed=5, remote_per_kind=2, measure_every=5)
+    mixed = json.loads(W.random_mixed(7, 150, seed=9).model_dump_json())
+    ins = mixed["instructions"]
+    for pos, q in ((40, 2), (90, 5), (120, 0)):  # measure, reset, conditional on the measured bit
+        ins.insert(pos, {"kind": "measure", "qubit": q, "cbit": 0})
+        ins.insert(pos + 1, {"kind": "reset", "qubit": (q + 1) % 7})
+        ins.insert(pos + 2, {"kind": "conditional", "condition": {"creg_base": 0, "creg_size": 1, "creg_value": 1},
+                             "op": {"kind": "cx", "control": q, "target": (q + 3) % 7}})
+    mixed["cregs"] = {"c": {"name": "c", "size": 1, "base": 0}}
+    yield "mixed", _JsonCircuit(mixed)
+
+
+@pytest.mark.parametrize("fusion", [1, 2])
+def test_profile_counts_equal_reference_rule(fusion):
+    """apply_one_qubit / apply_n_qubit / measure_qubit calls equal the counts
+    of the oracle's dispatcher, which follows the reference's profiling
+    wrappers (simulator.rs:890-946): a reset's X is a one-qubit call, every
+    2-5 qubit kind and generic gate an n-qubit call, remote_barrier /
+    remote_cu1 / id / u0 / barrier none."""
+    for name, circ in _profile_circuits():
+        sim = OracleSimulator(seed=5)
+        ref = sim.simulate(circ)
+        r = dq.StatevectorSimulator(seed=5, profile=True, fusion=fusion).simulate(circ)
+        p = r.profile
+        got = {"apply_one_qubit": p.apply_one_qubit_calls, "apply_n_qubit": p.apply_n_qubit_calls,
+               "measure_qubit": p.measure_qubit_calls}
+        assert got == sim.calls, name
+        assert r.classical_bits == ref.classical_bits, name
+        assert p.total_time > 0
+        assert p.total_time >= p.apply_one_qubit_time + p.apply_n_qubit_time + p.measure_qubit_time - 1e-9
     assert dq.StatevectorSimulator(seed=5).simulate(circ).profile is None
 
 
@@ -145,3 +179,32 @@ def test_bad_qubit_is_value_error():
     c = C.circuit(2, [C.HInstruction(qubit=5)])
     with pytest.raises(ValueError):
         dq.StatevectorSimulator(seed=1).simulate(c)
+
+
+@pytest.mark.parametrize("qubits", [[3, 9, 0], [2, 2, 5], [10], [7, 1, 7], [6, 63, 1, 4]])
+def test_probabilities_out_of_range_and_repeated_qubits(qubits):
+    """probabilities() / counts() take the reference's marginal_probs
+    semantics (src/engine.rs:368-390) for qubits at or above n (their set
+    bins are empty) and for repeated qubits; checked against the oracle's
+    literal restatement of that loop."""
+    from oracle import oracle as orc
+
+    n = 8
+    circ = W.random_mixed(n, 60, seed=21)
+    r, o = both(circ)
+    want = orc.marginal_probs(o.statevector, n, qubits, literal=True)
+    got = r.probabilities(qubits)
+    assert set(got) == {j for j, p in enumerate(want) if p > 0.0}
+    for j, p in got.items():
+        assert abs(p - want[j]) <= 1e-14 * max(1.0, want[j])
+    assert np.allclose(r.probabilities_array(qubits), want, rtol=1e-14, atol=1e-15)
+    c = r.counts(2000, qubits=qubits, seed=4)
+    assert c == orc.sample_counts(want, len(qubits), 2000, orc.Rng(4))
+
+
+def test_probabilities_negative_qubit_is_overflow_error():
+    r = dq.StatevectorSimulator(seed=1).simulate(C.circuit(2, [C.HInstruction(qubit=0)]))
+    with pytest.raises(OverflowError):
+        r.probabilities([-1])
+    with pytest.raises(ValueError):
+        r.counts(10, qubits=[64])

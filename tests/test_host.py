@@ -182,3 +182,35 @@ def test_product_does_not_import_oracle():
             if f.endswith((".py", ".cpp", ".hpp", ".hip", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.replace("the oracle", ""), f
+
+
+def test_profile_repr_format():
+    """SimulationProfile.__repr__ on fixed values, written out from the
+    reference's format string (simulator.rs:72-87: {:.2} ms total, {:4}
+    calls, {:8.2} ms, {:.1}%; total floored at 1e-9 s)."""
+    got = _core._profile_repr(3, 0.001, 2, 0.0025, 1, 0.0005, 0.005)
+    assert got == ("SimulationProfile (total: 5.00 ms)\n"
+                   "  apply_one_qubit :    3 calls      1.00 ms  (20.0%)\n"
+                   "  apply_n_qubit   :    2 calls      2.50 ms  (50.0%)\n"
+                   "  measure_qubit   :    1 calls      0.50 ms  (10.0%)")
+    # widths are minimums; a zero total divides by 1e-9
+    got = _core._profile_repr(12345, 0.0, 0, 0.0, 7, 123.456789, 0.0)
+    assert got == ("SimulationProfile (total: 0.00 ms)\n"
+                   "  apply_one_qubit : 12345 calls      0.00 ms  (0.0%)\n"
+                   "  apply_n_qubit   :    0 calls      0.00 ms  (0.0%)\n"
+                   f"  measure_qubit   :    7 calls  123456.79 ms  ({100.0 * 123.456789 / 1e-9:.1f}%)")
+
+
+def test_dqsim_import_path():
+    """The reference's own import line works (dqsim/__init__.py:1-27) and
+    names the engine's classes."""
+    import dqsim
+
+    for name in ("StatevectorSimulator", "StabilizerSimulator", "MpsSimulator", "SimulationResult",
+                 "SimulationProfile", "PBlockSimulator", "PBlockResult", "simulate_monolithic",
+                 "simulate_distributed", "simulate_monolithic_shots", "simulate_distributed_shots"):
+        assert name in dqsim.__all__ and hasattr(dqsim, name)
+    assert dqsim.StatevectorSimulator is dq.StatevectorSimulator
+    assert dqsim.simulate_monolithic is dq.simulate_monolithic
+    with pytest.raises(NotImplementedError):
+        dqsim.StabilizerSimulator(seed=1)

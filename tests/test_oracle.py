@@ -398,3 +398,66 @@ def test_reference_golden_cases():
                 assert abs(got.get(k, 0.0) - want.get(k, 0.0)) < case.get("tol", 1e-6), case["name"]
         if "classical_bits" in case:
             assert r.classical_bits == {int(k): v for k, v in case["classical_bits"].items()}, case["name"]
+
+
+def profile_case():
+    """A circuit whose engine calls follow from the reference's dispatcher by
+    hand (simulator.rs:394-632 with the do_oq / do_nq / do_mq wrappers of
+    :890-946): x -> oq; id, u0 -> none; cx -> nq; generic remote_barrier and
+    remote_cu1 -> none; generic nonlocal_cz -> nq; ccx -> nq; measure -> mq;
+    reset -> mq, plus oq (its X) when the outcome is 1; a conditional whose
+    test holds -> its op's call; one that fails -> none; barrier -> none."""
+    ins = [
+        {"kind": "x", "qubit": 0},                                     # oq 1
+        {"kind": "id", "qubit": 1},
+        {"kind": "u0", "qubit": 1},
+        {"kind": "cx", "control": 0, "target": 1},                     # nq 1   state |011>
+        {"kind": "gate", "name": "remote_barrier", "params": [], "qubits": [0, 1]},
+        {"kind": "gate", "name": "Remote_CU1", "params": [], "qubits": [0, 1]},
+        {"kind": "gate", "name": "nonlocal_cz", "params": [], "qubits": [0, 1]},   # nq 2
+        {"kind": "ccx", "control1": 0, "control2": 1, "target": 2},    # nq 3   state |111>
+        {"kind": "measure", "qubit": 2, "cbit": 0},                    # mq 1   outcome 1
+        {"kind": "reset", "qubit": 0},                                 # mq 2 + oq 2 (outcome 1)
+        {"kind": "reset", "qubit": 0},                                 # mq 3   (outcome 0)
+        {"kind": "conditional", "condition": {"creg_base": 0, "creg_size": 1, "creg_value": 1},
+         "op": {"kind": "h", "qubit": 1}},                             # oq 3
+        {"kind": "conditional", "condition": {"creg_base": 0, "creg_size": 1, "creg_value": 0},
+         "op": {"kind": "x", "qubit": 1}},
+        {"kind": "barrier"},
+    ]
+    circ = {"qregs": {"q": {"name": "q", "base": 0, "size": 3}}, "cregs": {"c": {"name": "c", "base": 0, "size": 1}},
+            "instructions": ins}
+    return circ, {"apply_one_qubit": 3, "apply_n_qubit": 3, "measure_qubit": 3}
+
+
+def test_profile_call_counts_known_answer():
+    circ, want = profile_case()
+    sim = orc.OracleSimulator(seed=3)
+    res = sim.simulate_json(json.dumps(circ))
+    assert sim.calls == want
+    assert res.classical_bits == {0: 1}
+
+
+def test_marginal_literal_out_of_range_and_repeated_qubits():
+    """engine.rs:368-390 builds mask and desired_j with 1 << q for every
+    listed qubit: a qubit at or above n leaves the bins that want it set
+    empty, and a repeated qubit makes every j that sets it anywhere want it
+    set.  Checked against numpy sums over the basis."""
+    rng = np.random.default_rng(4)
+    n = 5
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    psi /= np.linalg.norm(psi)
+    p = np.abs(psi) ** 2
+    idx = np.arange(1 << n)
+    bit = lambda q: (idx >> q) & 1  # noqa: E731
+    got = orc.marginal_probs(psi, n, [1, 7], literal=True)  # qubit 7 >= n
+    want = [p[bit(1) == 0].sum(), 0.0, p[bit(1) == 1].sum(), 0.0]
+    assert np.allclose(got, want, atol=1e-15, rtol=0)
+    got = orc.marginal_probs(psi, n, [2, 2], literal=True)
+    want = [p[bit(2) == 0].sum()] + [p[bit(2) == 1].sum()] * 3
+    assert np.allclose(got, want, atol=1e-15, rtol=0)
+    got = orc.marginal_probs(psi, n, [3, 0, 3], literal=True)
+    for j in range(8):
+        b3 = (j >> 2) & 1 | j & 1
+        b0 = (j >> 1) & 1
+        assert abs(got[j] - p[(bit(3) == b3) & (bit(0) == b0)].sum()) < 1e-15
