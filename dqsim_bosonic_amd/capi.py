@@ -20,7 +20,8 @@ ERR_ARG, ERR_OOM, ERR_DEVICE = -1, -2, -3
 # every function declared in include/dqsim_gpu.h
 EXPORTS = (
     "dq_state_create", "dq_state_destroy", "dq_state_num_qubits", "dq_state_set", "dq_state_reset",
-    "dq_apply_1q", "dq_apply_nq", "dq_prob_one", "dq_collapse", "dq_marginal_probs", "dq_inner",
+    "dq_apply_1q", "dq_apply_nq", "dq_prob_one", "dq_collapse", "dq_marginal_probs",
+    "dq_marginal_probs_noalloc", "dq_inner",
     "dq_copy_to_host", "dq_synchronize", "dq_state_copy", "dq_apply_pauli_sum", "dq_pauli_inner",
     "dq_compare_states", "dq_state_tensor", "dq_adjoint_sweep",
     "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
@@ -57,6 +58,7 @@ def lib() -> ctypes.CDLL:
         L.dq_prob_one.argtypes = [vp, ctypes.c_int, _dp]
         L.dq_collapse.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.dq_marginal_probs.argtypes = [vp, _ip, ctypes.c_int, _dp]
+        L.dq_marginal_probs_noalloc.argtypes = [vp, _ip, ctypes.c_int, _dp]
         L.dq_inner.argtypes = [vp, _dp, _dp, _dp]
         L.dq_copy_to_host.argtypes = [vp, _dp]
         L.dq_synchronize.argtypes = [vp]
@@ -171,6 +173,13 @@ class State:
         q = (ctypes.c_int * max(1, len(qubits)))(*qubits)
         out = np.zeros(1 << len(qubits), dtype=np.float64)
         _check(lib().dq_marginal_probs(self.h, q, len(qubits), _d(out)))
+        return out
+
+    def marginal_probs_noalloc(self, qubits) -> np.ndarray:
+        """dq_marginal_probs_noalloc: same values, no device allocation."""
+        q = (ctypes.c_int * max(1, len(qubits)))(*qubits)
+        out = np.zeros(1 << len(qubits), dtype=np.float64)
+        _check(lib().dq_marginal_probs_noalloc(self.h, q, len(qubits), _d(out)))
         return out
 
     def inner(self, other: np.ndarray) -> complex:

@@ -31,6 +31,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/dqsim_gpu.h"
@@ -1109,7 +1110,17 @@ int dq_prob_one(dq_state *s, int qubit, double *p1) {
     return DQ_OK;
 }
 
-int dq_marginal_probs(dq_state *s, const int *qubits, int k, double *out) {
+}  // extern "C"
+
+// Work space of dq_marginal_probs_noalloc (see below); 4 MiB + 4 KiB holds
+// the marginal over up to 18 qubits (19 when each bin sums <= 64 amplitudes).
+constexpr size_t kNoallocScratchBytes = (size_t(4) << 20) + 4096;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+// Argument checks shared by the marginal entry points.
+int check_marginal_args(dq_state *s, const int *qubits, int k, const double *out) {
     if (!s || !out || (k > 0 && !qubits)) return fail(DQ_ERR_ARG, "null argument");
     if (k < 0 || k > s->n) return fail(DQ_ERR_ARG, "number of qubits out of range");
     if (k > 30) return fail(DQ_ERR_ARG, "marginal over more than 30 qubits is not supported");
@@ -1120,32 +1131,54 @@ int dq_marginal_probs(dq_state *s, const int *qubits, int k, double *out) {
         if (seen & m) return fail(DQ_ERR_ARG, "qubits must be distinct");
         seen |= m;
     }
-    if (int rc = flush(s)) return rc;
+    return DQ_OK;
+}
+
+// Device work space of one marginal: the qubit table (128 ints), the output
+// bins and, when a bin has more than 64 terms, per-(bin, chunk) partials.
+struct MarginalShape {
+    uint64_t bins, terms;
+    int chunks;
+    size_t meta_b, out_b, part_b;
+    size_t bytes() const { return meta_b + out_b + part_b; }
+};
+
+MarginalShape marginal_shape(int n, int k) {
+    MarginalShape m;
+    m.bins = uint64_t(1) << k;
+    m.terms = uint64_t(1) << (n - k);
+    m.chunks = 1;
+    if (m.terms > 64) {
+        // enough blocks to fill the device, at least 256 terms per block
+        uint64_t want = std::max<uint64_t>(1, 4096 / m.bins);
+        while (uint64_t(m.chunks) * 2 <= want && m.terms / (uint64_t(m.chunks) * 2) >= 256) m.chunks *= 2;
+    }
+    m.meta_b = 128 * sizeof(int);
+    m.out_b = m.bins * sizeof(double);
+    m.part_b = (m.terms > 64) ? m.bins * uint64_t(m.chunks) * sizeof(double) : 0;
+    return m;
+}
+
+// The marginal of a flushed state into `out` (host), with `dbuf` (device,
+// at least shape.bytes(), 256-byte aligned) as the work space.  Every bin
+// and partial it reads is written first in the same call, so dbuf needs no
+// clearing.  Synchronises the state's stream before returning.
+int marginal_into(dq_state *s, const int *qubits, int k, double *out, void *dbuf, const MarginalShape &sh) {
     const int n = s->n;
-    const uint64_t bins = uint64_t(1) << k;
-    std::vector<int> meta(64 + 64);
-    for (int j = 0; j < k; ++j) meta[j] = qubits[j];
+    int meta[128] = {0};
+    uint64_t seen = 0;
+    for (int j = 0; j < k; ++j) {
+        meta[j] = qubits[j];
+        seen |= uint64_t(1) << qubits[j];
+    }
     int nrest = 0;
     for (int q = 0; q < n; ++q)
         if (!((seen >> q) & 1)) meta[64 + nrest++] = q;
-    // device buffers: meta (ints) + output bins (+ partials)
-    const uint64_t terms = uint64_t(1) << (n - k);
-    int chunks = 1;
-    if (terms > 64) {
-        // enough blocks to fill the device, at least 256 terms per block
-        uint64_t want = std::max<uint64_t>(1, 4096 / bins);
-        while (uint64_t(chunks) * 2 <= want && terms / (uint64_t(chunks) * 2) >= 256) chunks *= 2;
-    }
-    const size_t meta_b = 128 * sizeof(int);
-    const size_t out_b = bins * sizeof(double);
-    const size_t part_b = (terms > 64) ? bins * chunks * sizeof(double) : 0;
-    void *dbuf = nullptr;
-    if (hipMalloc(&dbuf, meta_b + out_b + part_b) != hipSuccess)
-        return fail(DQ_ERR_OOM, "hipMalloc for marginal probabilities failed");
     int *d_meta = static_cast<int *>(dbuf);
-    double *d_out = reinterpret_cast<double *>(static_cast<unsigned char *>(dbuf) + meta_b);
-    double *d_part = reinterpret_cast<double *>(static_cast<unsigned char *>(dbuf) + meta_b + out_b);
-    hipError_t e = hipMemcpyAsync(d_meta, meta.data(), meta_b, hipMemcpyHostToDevice, s->stream);
+    double *d_out = reinterpret_cast<double *>(static_cast<unsigned char *>(dbuf) + sh.meta_b);
+    double *d_part = reinterpret_cast<double *>(static_cast<unsigned char *>(dbuf) + sh.meta_b + sh.out_b);
+    const uint64_t bins = sh.bins;
+    hipError_t e = hipMemcpyAsync(d_meta, meta, sh.meta_b, hipMemcpyHostToDevice, s->stream);
     bool identity = (k == n);
     for (int j = 0; j < k && identity; ++j) identity = (qubits[j] == n - 1 - j);
     if (e == hipSuccess) {
@@ -1156,7 +1189,7 @@ int dq_marginal_probs(dq_state *s, const int *qubits, int k, double *out) {
             else
                 hipLaunchKernelGGL(k_probs_perm<float>, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream,
                                    static_cast<const cplx<float> *>(s->amps), n, d_meta, d_out, int(identity));
-        } else if (terms <= 64) {
+        } else if (sh.terms <= 64) {
             if (s->dtype == DQ_C128)
                 hipLaunchKernelGGL(k_marginal_seq<double>, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream,
                                    static_cast<const cplx<double> *>(s->amps), n, d_meta, k, d_meta + 64, d_out);
@@ -1164,26 +1197,91 @@ int dq_marginal_probs(dq_state *s, const int *qubits, int k, double *out) {
                 hipLaunchKernelGGL(k_marginal_seq<float>, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream,
                                    static_cast<const cplx<float> *>(s->amps), n, d_meta, k, d_meta + 64, d_out);
         } else {
-            const unsigned nblk = unsigned(bins * chunks);
+            const unsigned nblk = unsigned(bins * uint64_t(sh.chunks));
             if (s->dtype == DQ_C128)
                 hipLaunchKernelGGL(k_marginal_tree<double>, dim3(nblk), dim3(kThreads), 0, s->stream,
-                                   static_cast<const cplx<double> *>(s->amps), n, d_meta, k, d_meta + 64, chunks,
+                                   static_cast<const cplx<double> *>(s->amps), n, d_meta, k, d_meta + 64, sh.chunks,
                                    d_part);
             else
                 hipLaunchKernelGGL(k_marginal_tree<float>, dim3(nblk), dim3(kThreads), 0, s->stream,
-                                   static_cast<const cplx<float> *>(s->amps), n, d_meta, k, d_meta + 64, chunks,
+                                   static_cast<const cplx<float> *>(s->amps), n, d_meta, k, d_meta + 64, sh.chunks,
                                    d_part);
-            hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream, d_part, chunks,
+            hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(bins)), dim3(kThreads), 0, s->stream, d_part, sh.chunks,
                                bins, d_out);
         }
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, out_b, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sh.out_b, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    (void)hipFree(dbuf);
-    if (e != hipSuccess) return fail(DQ_ERR_DEVICE, std::string("marginal_probs: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s->stream);
+        return fail(DQ_ERR_DEVICE, std::string("marginal_probs: ") + hipGetErrorString(e));
+    }
     s->launches += 2;
     return DQ_OK;
+}
+
+// ---- the allocation-free marginal (dq_marginal_probs_noalloc) ----
+//
+// Its work space is this file-scope device array, one copy per GPU for the
+// life of the process.  Host code never passes the array's name to a kernel
+// or a copy (on the host that name is only a stand-in): the array's device
+// address is looked up once per device, with that device current, and kept
+// in noalloc_ptr.  Calls on one device are serialised by noalloc_mu, since
+// they share the array.
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mu[kMaxDevices];
+void *noalloc_ptr[kMaxDevices] = {nullptr};
+
+// Device address of `scratch` on the state's device (looked up on the first
+// call there; the caller holds noalloc_mu[device]).
+int noalloc_scratch(dq_state *s, void **ptr) {
+    const int dev = s->device;
+    if (!noalloc_ptr[dev]) {
+        int cur = 0;
+        HIP_TRY(hipGetDevice(&cur));
+        if (cur != dev) HIP_TRY(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) HIP_TRY(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return fail(DQ_ERR_DEVICE, std::string("hipGetSymbolAddress of the no-allocation work space: ") +
+                                           hipGetErrorString(e));
+        noalloc_ptr[dev] = p;
+    }
+    *ptr = noalloc_ptr[dev];
+    return DQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dq_marginal_probs(dq_state *s, const int *qubits, int k, double *out) {
+    if (int rc = check_marginal_args(s, qubits, k, out)) return rc;
+    if (int rc = flush(s)) return rc;
+    const MarginalShape sh = marginal_shape(s->n, k);
+    void *dbuf = nullptr;
+    if (hipMalloc(&dbuf, sh.bytes()) != hipSuccess) return fail(DQ_ERR_OOM, "hipMalloc for marginal probabilities failed");
+    const int rc = marginal_into(s, qubits, k, out, dbuf, sh);
+    (void)hipFree(dbuf);
+    return rc;
+}
+
+int dq_marginal_probs_noalloc(dq_state *s, const int *qubits, int k, double *out) {
+    if (int rc = check_marginal_args(s, qubits, k, out)) return rc;
+    if (!s->queue.empty())
+        return fail(DQ_ERR_ARG, "marginal_probs_noalloc: gates are still queued; call dq_synchronize first");
+    const MarginalShape sh = marginal_shape(s->n, k);
+    if (sh.bytes() > kNoallocScratchBytes)
+        return fail(DQ_ERR_ARG, "marginal_probs_noalloc: " + std::to_string(k) + " qubits need " +
+                                    std::to_string(sh.bytes()) + " bytes of work space, more than its maximum of " +
+                                    std::to_string(kNoallocScratchBytes));
+    if (s->device < 0 || s->device >= kMaxDevices) return fail(DQ_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(noalloc_mu[s->device]);
+    void *dbuf = nullptr;
+    if (int rc = noalloc_scratch(s, &dbuf)) return rc;
+    return marginal_into(s, qubits, k, out, dbuf, sh);
 }
 
 int dq_inner(dq_state *s, const double *other, double *re, double *im) {

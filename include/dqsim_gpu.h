@@ -81,6 +81,20 @@ int dq_collapse(dq_state *s, int qubit, int outcome, double norm);
  * (src/engine.rs:368-390) behind SimulationResult.probabilities/counts. */
 int dq_marginal_probs(dq_state *s, const int *qubits, int k, double *out);
 
+/* dq_marginal_probs without any device allocation: no hipMalloc, hipFree
+ * or allocating library call.  No counterpart in the reference; it is for
+ * callers that cannot allocate (code under graph capture, tight serving
+ * loops, harnesses that forbid it).  Its work space is a fixed array of
+ * 4 MiB + 4 KiB in the library's device memory, one per GPU, held until the
+ * process ends: a call needs 512 + 8 * 2^k bytes, plus as much again for
+ * partial sums when a bin adds up more than 64 amplitudes, so k <= 18 always
+ * fits and k = 19 fits when n - k <= 6 (DQ_ERR_ARG beyond).  The state must have
+ * no queued gates (DQ_ERR_ARG otherwise: call dq_synchronize first).  Calls
+ * on one GPU share the array and are serialised by a lock; `out` (host, 2^k
+ * doubles) is written before the call returns.  Same values as
+ * dq_marginal_probs. */
+int dq_marginal_probs_noalloc(dq_state *s, const int *qubits, int k, double *out);
+
 /* sum_i conj(psi_i) * other_i with `other` a host array of 2^n (re,im)
  * doubles.  Replaces the fold in SimulationResult.fidelity
  * (src/monolithic/statevector/simulator.rs:184-191). */
