@@ -25,7 +25,7 @@ EXPORTS = (
     "dq_copy_to_host", "dq_synchronize", "dq_state_copy", "dq_apply_pauli_sum", "dq_pauli_inner",
     "dq_compare_states", "dq_state_tensor", "dq_adjoint_sweep",
     "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
-    "dq_get_fused_stats", "dq_set_fuse_min_cost", "dq_set_pair_fusion", "dq_top_k",
+    "dq_get_fused_stats", "dq_set_fuse_min_cost", "dq_set_pair_fusion", "dq_set_reg_bits", "dq_top_k",
     "dq_batch_create", "dq_batch_destroy", "dq_batch_add_instance", "dq_batch_apply",
     "dq_batch_set_observable", "dq_batch_set_workers", "dq_batch_run", "dq_batch_stats",
     "dq_shots_create", "dq_shots_destroy", "dq_shots_add_gate", "dq_shots_add_measure", "dq_shots_add_reset",
@@ -68,6 +68,7 @@ def lib() -> ctypes.CDLL:
         L.dq_get_fused_stats.argtypes = [vp, u64p, u64p]
         L.dq_set_fuse_min_cost.argtypes = [vp, ctypes.c_double]
         L.dq_set_pair_fusion.argtypes = [vp, ctypes.c_int]
+        L.dq_set_reg_bits.argtypes = [vp, ctypes.c_int]
         L.dq_top_k.argtypes = [vp, ctypes.c_uint64, u64p, _dp]
         L.dq_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
         L.dq_batch_destroy.argtypes = [vp]
@@ -242,6 +243,9 @@ class State:
 
     def set_pair_fusion(self, on: bool) -> None:
         _check(lib().dq_set_pair_fusion(self.h, int(bool(on))))
+
+    def set_reg_bits(self, r: int) -> None:
+        _check(lib().dq_set_reg_bits(self.h, int(r)))
 
     def set_fuse_min_cost(self, min_cost: float) -> None:
         _check(lib().dq_set_fuse_min_cost(self.h, float(min_cost)))

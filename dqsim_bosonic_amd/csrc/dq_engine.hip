@@ -504,6 +504,9 @@ PlanConfig plan_config(const dq_state *s) {
     c.reg_tiles = s->reg_tiles;
     c.fuse_min_cost = s->fuse_min_cost;
     c.pair_fusion = s->pair_fusion;
+    // 8 amplitudes per thread for the contracted complex128 sweep; the exact
+    // (fusion level 1) and complex64 sweeps keep 16
+    c.reg_bits = (s->dtype == DQ_C128 && s->fuse >= 2) ? s->reg_bits : 4;
     return c;
 }
 
@@ -718,7 +721,8 @@ int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *ma
     // fusion level 1 keeps the reference's rounding: the exact kernel;
     // level 2 already reorders commuting ops and uses the contracted kernel.
     auto *fn = std::is_same<T, double>::value
-                   ? (s->fuse >= 2 ? dq_launch_tile_reg_f64_fma : dq_launch_tile_reg_f64)
+                   ? (s->fuse >= 2 ? (rp.reg_bits == 3 ? dq_launch_tile_reg8_f64_fma : dq_launch_tile_reg_f64_fma)
+                                   : dq_launch_tile_reg_f64)
                    : (rp.fused_blocks > 0 ? dq_launch_tile_reg_f32_mfma : dq_launch_tile_reg_f32);
     hipError_t e = fn(rp.b, unsigned(grid), s->stream, s->amps, rp.pd, ops, mats, ntiles, s->d_partials);
     s->last_grid = int(grid);
@@ -763,7 +767,7 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     std::vector<uint32_t> lays;
     for (RegPlan &rp : plans) {
         rp.pd.lay_off = int(mats_b + lays.size() * sizeof(uint32_t));
-        for (int rb : rp.layouts) append_layout_table(lays, rb, rp.b, unsigned(2 * sizeof(T)));
+        for (int rb : rp.layouts) append_layout_table(lays, rb, rp.b, unsigned(2 * sizeof(T)), rp.reg_bits);
     }
     const size_t lays_b = align_up(std::max<size_t>(lays.size(), 1) * sizeof(uint32_t), 256);
     const size_t total = ops_b + mats_b + lays_b;
@@ -1767,6 +1771,14 @@ int dq_set_pair_fusion(dq_state *s, int on) {
     if (on != 0 && on != 1) return fail(DQ_ERR_ARG, "pair fusion flag must be 0 or 1");
     if (int rc = flush(s)) return rc;
     s->pair_fusion = on;
+    return DQ_OK;
+}
+
+int dq_set_reg_bits(dq_state *s, int reg_bits) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (reg_bits != 3 && reg_bits != 4) return fail(DQ_ERR_ARG, "register bits must be 3 or 4");
+    if (int rc = flush(s)) return rc;
+    s->reg_bits = reg_bits;
     return DQ_OK;
 }
 

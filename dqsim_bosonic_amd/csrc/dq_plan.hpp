@@ -26,6 +26,7 @@ struct PlanConfig {
     int reg_tiles = 1;   // register-resident tiles for tiles of >= 9 qubits
     double fuse_min_cost = 128.0;  // fusion level 3: least cost of a fused block's gates
     int pair_fusion = 1;           // fusion levels 2-3: merge neighbouring gates on <= 2 qubits
+    int reg_bits = 4;              // register bits of a register-tile layout (4: 16, 3: 8 amplitudes per thread)
 };
 
 inline bool dq_tile_bits_supported(int b) { return b == 9 || b == 12 || b == 13; }
@@ -583,21 +584,22 @@ inline std::vector<const HostOp *> fuse_pairs(const std::vector<const HostOp *> 
 // ---- register-resident tile passes (k_tile_reg) ----
 
 
-inline int pack_rb(unsigned mask) {
+// A layout of r register bits is packed as r 4-bit tile-bit numbers, ascending.
+inline int pack_rb(unsigned mask, int r = 4) {
     int packed = 0, m = 0;
-    for (int bpos = 0; bpos < 16 && m < 4; ++bpos)
+    for (int bpos = 0; bpos < 16 && m < r; ++bpos)
         if ((mask >> bpos) & 1) packed |= bpos << (4 * m++);
     return packed;
 }
 
-inline unsigned unpack_rb(int packed) {
+inline unsigned unpack_rb(int packed, int r = 4) {
     unsigned m = 0;
-    for (int j = 0; j < 4; ++j) m |= 1u << ((packed >> (4 * j)) & 15);
+    for (int j = 0; j < r; ++j) m |= 1u << ((packed >> (4 * j)) & 15);
     return m;
 }
 
-inline int rb_slot(int packed, int tile_bit) {
-    for (int j = 0; j < 4; ++j)
+inline int rb_slot(int packed, int tile_bit, int r = 4) {
+    for (int j = 0; j < r; ++j)
         if (((packed >> (4 * j)) & 15) == tile_bit) return j;
     return -1;
 }
@@ -633,6 +635,7 @@ inline std::vector<double> diag_to_dense(const std::vector<double> &d) {
 struct RegPlan {
     TilePass pd;
     int b;
+    int reg_bits = 4;
     std::vector<TileOp> ops;
     std::vector<int> layouts;  // distinct layouts of the pass (entries of its layout table)
     int fused_blocks = 0;  // K_FUSED ops formed in this pass
@@ -650,7 +653,8 @@ struct RegPlan {
 // ops keep program order (the search is quadratic in the op count).
 constexpr size_t kOrderMaxOps = 64;
 
-inline std::vector<const HostOp *> order_for_layouts(const std::vector<const HostOp *> &in, const int *pos) {
+inline std::vector<const HostOp *> order_for_layouts(const std::vector<const HostOp *> &in, const int *pos,
+                                                     int reg_bits = 4) {
     const size_t n = in.size();
     if (n < 3 || n > kOrderMaxOps) return in;
     std::vector<unsigned> need(n, 0), soft(n, 0), qm(n, 0);
@@ -712,7 +716,7 @@ inline std::vector<const HostOp *> order_for_layouts(const std::vector<const Hos
                     if (done[j] || j == k) continue;
                     const unsigned nd = need[j] ? need[j] : (mode ? soft[j] : 0u);
                     if (!nd || (nd & ~S) == 0) continue;
-                    if (__builtin_popcount(S | nd) <= 4) S |= nd;
+                    if (__builtin_popcount(S | nd) <= reg_bits) S |= nd;
                     else if (need[j] && mode == 0) break;
                 }
                 if (std::find(cands.begin(), cands.end(), S) == cands.end()) cands.push_back(S);
@@ -746,12 +750,12 @@ inline unsigned tile_swizzle(unsigned i, int b) {
     return i ^ s;
 }
 
-// One layout-table entry: for register j < 16 of layout rb, the LDS byte
+// One layout-table entry: for register j < 2^r of layout rb, the LDS byte
 // offset of its register bits, swizzled, for amplitudes of elem bytes.
-inline void append_layout_table(std::vector<uint32_t> &out, int rb, int b, unsigned elem) {
-    for (int j = 0; j < 16; ++j) {
+inline void append_layout_table(std::vector<uint32_t> &out, int rb, int b, unsigned elem, int r = 4) {
+    for (int j = 0; j < (1 << r); ++j) {
         unsigned v = 0;
-        for (int m = 0; m < 4; ++m)
+        for (int m = 0; m < r; ++m)
             if ((j >> m) & 1) v |= 1u << ((rb >> (4 * m)) & 15);
         out.push_back(tile_swizzle(v, b) * elem);
     }
@@ -776,11 +780,13 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     }
     const int b = tb;
     out.b = b;
+    const int R = s->reg_bits;  // register bits per layout
+    out.reg_bits = R;
     auto inner = [&](int bitpos) { return bitpos < kOuterBit; };
     pd.n_outer = ob;
     // the low qubits stay thread bits at load/store (coalesced runs)
-    const unsigned lowmask = (1u << std::max(0, std::min(s->low_bits, b - 4))) - 1;
-    const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - 4)) - 1);
+    const unsigned lowmask = (1u << std::max(0, std::min(s->low_bits, b - R))) - 1;
+    const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - R)) - 1);
     std::deque<HostOp> fused_store, pair_store;
     std::vector<const HostOp *> eff;
     for (int idx : p.ops) eff.push_back(&queue[idx]);
@@ -792,7 +798,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
             out.fused_gates += f.outcome;
         }
     }
-    if (s->fuse >= 2) eff = order_for_layouts(eff, pos);
+    if (s->fuse >= 2) eff = order_for_layouts(eff, pos, R);
     const size_t nops = eff.size();
     std::vector<unsigned> hard(nops, 0), soft(nops, 0);
     for (size_t i = 0; i < nops; ++i) {
@@ -824,22 +830,22 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
         for (size_t j = i + 1; j < nops && j < i + 64; ++j) {
             const unsigned need = hard[j] ? hard[j] : soft[j];
             if (!need || (need & ~S) == 0) continue;
-            if (__builtin_popcount(S | need) <= 4) S |= need;
+            if (__builtin_popcount(S | need) <= R) S |= need;
             else if (hard[j]) break;
         }
-        // fill to 4 bits: keep current register bits, then high tile bits, then the rest
-        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos)
+        // fill to R bits: keep current register bits, then high tile bits, then the rest
+        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < R; --bpos)
             if (((curmask >> bpos) & 1) && !((lowmask >> bpos) & 1)) S |= 1u << bpos;
-        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos)
+        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < R; --bpos)
             if (!((lowmask >> bpos) & 1)) S |= 1u << bpos;
-        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < 4; --bpos) S |= 1u << bpos;
+        for (int bpos = b - 1; bpos >= 0 && __builtin_popcount(S) < R; --bpos) S |= 1u << bpos;
         return S;
     };
     std::vector<int> lay(nops, -1), relay(nops, 0);
     int cur = -1;
     for (size_t i = 0; i < nops; ++i) {
-        if (hard[i] && (cur < 0 || (hard[i] & ~unpack_rb(cur)))) {
-            cur = pack_rb(build(i, cur < 0 ? 0u : unpack_rb(cur)));
+        if (hard[i] && (cur < 0 || (hard[i] & ~unpack_rb(cur, R)))) {
+            cur = pack_rb(build(i, cur < 0 ? 0u : unpack_rb(cur, R)), R);
             relay[i] = 1;
         }
         lay[i] = cur;
@@ -848,15 +854,15 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     size_t first = nops;
     for (size_t i = 0; i < nops; ++i)
         if (relay[i]) { first = i; break; }
-    if (first < nops && !(unpack_rb(lay[first]) & lowmask)) {
+    if (first < nops && !(unpack_rb(lay[first], R) & lowmask)) {
         pd.load_rb = lay[first];
         relay[first] = 0;
     } else {
-        pd.load_rb = pack_rb(dflt);
+        pd.load_rb = pack_rb(dflt, R);
     }
     for (size_t i = 0; i < first && i < nops; ++i) lay[i] = pd.load_rb;
     const int last = nops ? lay[nops - 1] : pd.load_rb;
-    pd.store_rb = (unpack_rb(last) & lowmask) ? pack_rb(dflt) : last;
+    pd.store_rb = (unpack_rb(last, R) & lowmask) ? pack_rb(dflt, R) : last;
     auto lay_index = [&](int rb) {
         for (size_t e = 0; e < out.layouts.size(); ++e)
             if (out.layouts[e] == rb) return int(e);
@@ -877,7 +883,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
         for (int j = 0; j < h.k; ++j) t.bit[j] = pos[h.q[j]];
         for (int j = 0; j < h.k; ++j) t.sbit[j] = t.bit[j];
         std::sort(t.sbit, t.sbit + h.k);
-        const unsigned regmask = unpack_rb(lay[i]);
+        const unsigned regmask = unpack_rb(lay[i], R);
         unsigned qm = 0;
         bool all_inner = true;
         for (int j = 0; j < h.k; ++j) {
@@ -892,12 +898,12 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
         case K_DENSEK:
             if (h.k == 1) {
                 t.kind = T_DENSE1;
-                t.p = rb_slot(lay[i], t.bit[0]);
+                t.p = rb_slot(lay[i], t.bit[0], R);
                 m = &h.dense;
             } else if (h.k == 2) {
                 t.kind = T_DENSE2;
-                t.p = rb_slot(lay[i], t.bit[0]);
-                t.q = rb_slot(lay[i], t.bit[1]);
+                t.p = rb_slot(lay[i], t.bit[0], R);
+                t.q = rb_slot(lay[i], t.bit[1], R);
                 m = &h.dense;
             } else {
                 t.kind = T_LDSOP;
@@ -906,7 +912,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
             break;
         case K_CTRL1:
             t.kind = T_CTRL1;
-            t.p = rb_slot(lay[i], t.bit[h.k - 1]);
+            t.p = rb_slot(lay[i], t.bit[h.k - 1], R);
             for (int j = 0; j < h.k - 1; ++j) {
                 const bool one = (h.cval & (uint64_t(1) << h.q[j])) != 0;
                 if (inner(t.bit[j])) {
@@ -923,8 +929,8 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
                 tmp = diag_to_dense(h.mat);
                 m = &tmp;
                 t.kind = h.k == 1 ? T_DENSE1 : T_DENSE2;
-                t.p = rb_slot(lay[i], t.bit[0]);
-                if (h.k == 2) t.q = rb_slot(lay[i], t.bit[1]);
+                t.p = rb_slot(lay[i], t.bit[0], R);
+                if (h.k == 2) t.q = rb_slot(lay[i], t.bit[1], R);
             } else if (h.k <= 2 && __builtin_popcount(qm & regmask) <= 1) {
                 // thread bits (and at most one register bit): T_DIAGT
                 t.kind = T_DIAGT;
@@ -932,7 +938,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
                 t.q = -1;
                 for (int j = 0; j < h.k; ++j)
                     if (inner(t.bit[j]) && ((regmask >> t.bit[j]) & 1)) {
-                        t.p = rb_slot(lay[i], t.bit[j]);
+                        t.p = rb_slot(lay[i], t.bit[j], R);
                         t.q = j;
                     }
             } else {

@@ -43,6 +43,7 @@ struct dq_state {
     uint64_t fused_blocks = 0, fused_gates = 0;  // fusion level 3 statistics
     double fuse_min_cost = 128.0;                // fusion level 3 cost threshold
     int pair_fusion = 1;                         // fusion levels 2-3: merge gates on <= 2 qubits
+    int reg_bits = 4;                            // complex128 register-tile layouts at fusion levels 2-3
     // optional per-launch timing of the sweep kernels (HIP events on `stream`)
     int timing = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;

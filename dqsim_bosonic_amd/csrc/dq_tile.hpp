@@ -21,16 +21,16 @@
 
 namespace dq {
 
-template <typename T>
+template <typename T, int R>
 struct Layout {
     unsigned tpart;  // tile-index bits contributed by this thread
-    int rb;          // the layout (register bits, 4 bits each, ascending)
+    int rb;          // the layout (R register bits, 4 bits each, ascending)
     // tile-index bits contributed by register j (uniform), computed where an
     // op needs them rather than held for the whole pass
     __device__ __forceinline__ unsigned jp(int j) const {
         unsigned v = 0;
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+        for (int m = 0; m < R; ++m)
             if ((j >> m) & 1) v |= 1u << ((rb >> (4 * m)) & 15);
         return v;
     }
@@ -44,18 +44,18 @@ __device__ __forceinline__ unsigned swz(unsigned i) {
     return i ^ s;
 }
 
-template <typename T, int B>
-__device__ __forceinline__ void make_layout(Layout<T> &L, int rb, unsigned t) {
-    int rbit[4];
+template <typename T, int B, int R>
+__device__ __forceinline__ void make_layout(Layout<T, R> &L, int rb, unsigned t) {
+    int rbit[R];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) rbit[m] = (rb >> (4 * m)) & 15;
+    for (int m = 0; m < R; ++m) rbit[m] = (rb >> (4 * m)) & 15;
     L.rb = rb;
     // thread bit c -> the c-th non-register tile bit, ascending: insert a
     // zero bit into t at each register bit, in ascending order (the host
     // packs the register bits of a layout ascending)
     unsigned tp = t;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
+    for (int m = 0; m < R; ++m) {
         const unsigned lo = (1u << rbit[m]) - 1;
         tp = (tp & lo) | ((tp & ~lo) << 1);
     }
@@ -147,11 +147,11 @@ __device__ __forceinline__ constexpr int pair_lo(int jj) {
 }
 
 // 2x2 on register slot P, pairs (j0, j0 | 1 << P); out = u00*x + u01*y, u10*x + u11*y.
-template <typename T, int P, int V, int FAST>
-__device__ __forceinline__ void reg_dense1(cplx<T> (&a)[16], const cplx<T> *__restrict__ m) {
+template <typename T, int P, int V, int FAST, int R>
+__device__ __forceinline__ void reg_dense1(cplx<T> (&a)[1 << R], const cplx<T> *__restrict__ m) {
     const cplx<T> u00 = m[0], u01 = m[1], u10 = m[2], u11 = m[3];
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
+    for (int jj = 0; jj < (1 << (R - 1)); ++jj) {
         const int j0 = pair_lo<P>(jj), j1 = j0 | (1 << P);
         const cplx<T> x = a[j0], y = a[j1];
         if (FAST && V == V_GEN) {
@@ -177,12 +177,12 @@ __device__ __forceinline__ void reg_dense1(cplx<T> (&a)[16], const cplx<T> *__re
 }
 
 // Controlled 2x2 on slot P: the pair changes only where the control pattern holds.
-template <typename T, int P, int V, int FAST>
-__device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[16], const cplx<T> *__restrict__ m, const Layout<T> &L,
+template <typename T, int P, int V, int FAST, int R>
+__device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[1 << R], const cplx<T> *__restrict__ m, const Layout<T, R> &L,
                                           unsigned cmask, unsigned cval) {
     const cplx<T> u00 = m[0], u01 = m[1], u10 = m[2], u11 = m[3];
 #pragma unroll
-    for (int jj = 0; jj < 8; ++jj) {
+    for (int jj = 0; jj < (1 << (R - 1)); ++jj) {
         const int j0 = pair_lo<P>(jj), j1 = j0 | (1 << P);
         const cplx<T> x = a[j0], y = a[j1];
         cplx<T> n0, n1;
@@ -203,11 +203,11 @@ __device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[16], const cplx<T> *__res
 }
 
 // 4x4 on register slots P (matrix MSB) and Q: acc = 0; acc += U[r][c] * v[c].
-template <typename T, int P, int Q, int V, int FAST>
-__device__ __forceinline__ void reg_dense2(cplx<T> (&a)[16], const cplx<T> *__restrict__ m) {
+template <typename T, int P, int Q, int V, int FAST, int R>
+__device__ __forceinline__ void reg_dense2(cplx<T> (&a)[1 << R], const cplx<T> *__restrict__ m) {
     constexpr int LO = P < Q ? P : Q, HI = P < Q ? Q : P;
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < (1 << (R - 2)); ++g) {
         int base = g;
         base = ((base >> LO) << (LO + 1)) | (base & ((1 << LO) - 1));
         base = ((base >> HI) << (HI + 1)) | (base & ((1 << HI) - 1));
@@ -241,68 +241,91 @@ __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[16], const cplx<T> *__re
     }
 }
 
-template <typename T, int P, int FAST>
-__device__ __forceinline__ void dense1_v(cplx<T> (&a)[16], int v, const cplx<T> *m) {
+template <typename T, int P, int FAST, int R>
+__device__ __forceinline__ void dense1_v(cplx<T> (&a)[1 << R], int v, const cplx<T> *m) {
     switch (v) {
-    case V_REAL: reg_dense1<T, P, V_REAL, FAST>(a, m); break;
-    case V_DIAG: reg_dense1<T, P, V_DIAG, FAST>(a, m); break;
-    case V_ANTI: reg_dense1<T, P, V_ANTI, FAST>(a, m); break;
-    default: reg_dense1<T, P, V_GEN, FAST>(a, m); break;
+    case V_REAL: reg_dense1<T, P, V_REAL, FAST, R>(a, m); break;
+    case V_DIAG: reg_dense1<T, P, V_DIAG, FAST, R>(a, m); break;
+    case V_ANTI: reg_dense1<T, P, V_ANTI, FAST, R>(a, m); break;
+    default: reg_dense1<T, P, V_GEN, FAST, R>(a, m); break;
     }
 }
 
-template <typename T, int FAST>
-__device__ __forceinline__ void dispatch_dense1(cplx<T> (&a)[16], int p, int v, const cplx<T> *m) {
+// Register slot p < R (the host packs only register bits of the layout).
+template <typename T, int FAST, int R>
+__device__ __forceinline__ void dispatch_dense1(cplx<T> (&a)[1 << R], int p, int v, const cplx<T> *m) {
     switch (p) {
-    case 0: dense1_v<T, 0, FAST>(a, v, m); break;
-    case 1: dense1_v<T, 1, FAST>(a, v, m); break;
-    case 2: dense1_v<T, 2, FAST>(a, v, m); break;
-    default: dense1_v<T, 3, FAST>(a, v, m); break;
+    case 0: dense1_v<T, 0, FAST, R>(a, v, m); break;
+    case 1: dense1_v<T, 1, FAST, R>(a, v, m); break;
+    case 2:
+        if constexpr (R > 3) {
+            dense1_v<T, 2, FAST, R>(a, v, m);
+            break;
+        }
+        [[fallthrough]];
+    default: dense1_v<T, R - 1, FAST, R>(a, v, m); break;
     }
 }
 
-template <typename T, int P, int FAST>
-__device__ __forceinline__ void ctrl1_v(cplx<T> (&a)[16], int v, const cplx<T> *m, const Layout<T> &L, unsigned cm,
-                                        unsigned cv) {
-    if (v == V_ANTI) reg_ctrl1<T, P, V_ANTI, FAST>(a, m, L, cm, cv);
-    else reg_ctrl1<T, P, V_GEN, FAST>(a, m, L, cm, cv);
+template <typename T, int P, int FAST, int R>
+__device__ __forceinline__ void ctrl1_v(cplx<T> (&a)[1 << R], int v, const cplx<T> *m, const Layout<T, R> &L,
+                                        unsigned cm, unsigned cv) {
+    if (v == V_ANTI) reg_ctrl1<T, P, V_ANTI, FAST, R>(a, m, L, cm, cv);
+    else reg_ctrl1<T, P, V_GEN, FAST, R>(a, m, L, cm, cv);
 }
 
-template <typename T, int FAST>
-__device__ __forceinline__ void dispatch_ctrl1(cplx<T> (&a)[16], int p, int v, const cplx<T> *m, const Layout<T> &L,
-                                               unsigned cmask, unsigned cval) {
+template <typename T, int FAST, int R>
+__device__ __forceinline__ void dispatch_ctrl1(cplx<T> (&a)[1 << R], int p, int v, const cplx<T> *m,
+                                               const Layout<T, R> &L, unsigned cmask, unsigned cval) {
     switch (p) {
-    case 0: ctrl1_v<T, 0, FAST>(a, v, m, L, cmask, cval); break;
-    case 1: ctrl1_v<T, 1, FAST>(a, v, m, L, cmask, cval); break;
-    case 2: ctrl1_v<T, 2, FAST>(a, v, m, L, cmask, cval); break;
-    default: ctrl1_v<T, 3, FAST>(a, v, m, L, cmask, cval); break;
+    case 0: ctrl1_v<T, 0, FAST, R>(a, v, m, L, cmask, cval); break;
+    case 1: ctrl1_v<T, 1, FAST, R>(a, v, m, L, cmask, cval); break;
+    case 2:
+        if constexpr (R > 3) {
+            ctrl1_v<T, 2, FAST, R>(a, v, m, L, cmask, cval);
+            break;
+        }
+        [[fallthrough]];
+    default: ctrl1_v<T, R - 1, FAST, R>(a, v, m, L, cmask, cval); break;
     }
 }
 
-template <typename T, int P, int Q, int FAST>
-__device__ __forceinline__ void dense2_v(cplx<T> (&a)[16], int v, const cplx<T> *m) {
+template <typename T, int P, int Q, int FAST, int R>
+__device__ __forceinline__ void dense2_v(cplx<T> (&a)[1 << R], int v, const cplx<T> *m) {
     switch (v) {
-    case V_REAL: reg_dense2<T, P, Q, V_REAL, FAST>(a, m); break;
-    case V_DIAG: reg_dense2<T, P, Q, V_DIAG, FAST>(a, m); break;
-    default: reg_dense2<T, P, Q, V_GEN, FAST>(a, m); break;
+    case V_REAL: reg_dense2<T, P, Q, V_REAL, FAST, R>(a, m); break;
+    case V_DIAG: reg_dense2<T, P, Q, V_DIAG, FAST, R>(a, m); break;
+    default: reg_dense2<T, P, Q, V_GEN, FAST, R>(a, m); break;
     }
 }
 
-template <typename T, int FAST>
-__device__ __forceinline__ void dispatch_dense2(cplx<T> (&a)[16], int p, int q, int v, const cplx<T> *m) {
-    switch (p * 4 + q) {
-    case 0 * 4 + 1: dense2_v<T, 0, 1, FAST>(a, v, m); break;
-    case 0 * 4 + 2: dense2_v<T, 0, 2, FAST>(a, v, m); break;
-    case 0 * 4 + 3: dense2_v<T, 0, 3, FAST>(a, v, m); break;
-    case 1 * 4 + 0: dense2_v<T, 1, 0, FAST>(a, v, m); break;
-    case 1 * 4 + 2: dense2_v<T, 1, 2, FAST>(a, v, m); break;
-    case 1 * 4 + 3: dense2_v<T, 1, 3, FAST>(a, v, m); break;
-    case 2 * 4 + 0: dense2_v<T, 2, 0, FAST>(a, v, m); break;
-    case 2 * 4 + 1: dense2_v<T, 2, 1, FAST>(a, v, m); break;
-    case 2 * 4 + 3: dense2_v<T, 2, 3, FAST>(a, v, m); break;
-    case 3 * 4 + 0: dense2_v<T, 3, 0, FAST>(a, v, m); break;
-    case 3 * 4 + 1: dense2_v<T, 3, 1, FAST>(a, v, m); break;
-    default: dense2_v<T, 3, 2, FAST>(a, v, m); break;
+// Slots p != q, both < R.
+template <typename T, int FAST, int R>
+__device__ __forceinline__ void dispatch_dense2(cplx<T> (&a)[1 << R], int p, int q, int v, const cplx<T> *m) {
+    if constexpr (R == 3) {
+        switch (p * 4 + q) {
+        case 0 * 4 + 1: dense2_v<T, 0, 1, FAST, R>(a, v, m); break;
+        case 0 * 4 + 2: dense2_v<T, 0, 2, FAST, R>(a, v, m); break;
+        case 1 * 4 + 0: dense2_v<T, 1, 0, FAST, R>(a, v, m); break;
+        case 1 * 4 + 2: dense2_v<T, 1, 2, FAST, R>(a, v, m); break;
+        case 2 * 4 + 0: dense2_v<T, 2, 0, FAST, R>(a, v, m); break;
+        default: dense2_v<T, 2, 1, FAST, R>(a, v, m); break;
+        }
+    } else {
+        switch (p * 4 + q) {
+        case 0 * 4 + 1: dense2_v<T, 0, 1, FAST, R>(a, v, m); break;
+        case 0 * 4 + 2: dense2_v<T, 0, 2, FAST, R>(a, v, m); break;
+        case 0 * 4 + 3: dense2_v<T, 0, 3, FAST, R>(a, v, m); break;
+        case 1 * 4 + 0: dense2_v<T, 1, 0, FAST, R>(a, v, m); break;
+        case 1 * 4 + 2: dense2_v<T, 1, 2, FAST, R>(a, v, m); break;
+        case 1 * 4 + 3: dense2_v<T, 1, 3, FAST, R>(a, v, m); break;
+        case 2 * 4 + 0: dense2_v<T, 2, 0, FAST, R>(a, v, m); break;
+        case 2 * 4 + 1: dense2_v<T, 2, 1, FAST, R>(a, v, m); break;
+        case 2 * 4 + 3: dense2_v<T, 2, 3, FAST, R>(a, v, m); break;
+        case 3 * 4 + 0: dense2_v<T, 3, 0, FAST, R>(a, v, m); break;
+        case 3 * 4 + 1: dense2_v<T, 3, 1, FAST, R>(a, v, m); break;
+        default: dense2_v<T, 3, 2, FAST, R>(a, v, m); break;
+        }
     }
 }
 
@@ -313,8 +336,8 @@ __device__ __forceinline__ void dispatch_dense2(cplx<T> (&a)[16], int p, int q, 
 // from the host as byte offsets (the layout table of the pass, one entry of
 // 16 per layout), so each LDS access costs one XOR: computed in the kernel
 // they were folded back into a full per-lane swizzle of every index.
-template <typename T, int B>
-__device__ __forceinline__ unsigned lds_base(const Layout<T> &L) {
+template <typename T, int B, int R>
+__device__ __forceinline__ unsigned lds_base(const Layout<T, R> &L) {
     return swz<B>(L.tpart) * unsigned(sizeof(cplx<T>));
 }
 
@@ -323,11 +346,11 @@ __device__ __forceinline__ unsigned lds_base(const Layout<T> &L) {
 template <typename T>
 using amp_vec = T __attribute__((ext_vector_type(2)));
 
-template <typename T>
-__device__ __forceinline__ void to_lds(unsigned char *lds, const cplx<T> (&a)[16], unsigned st,
+template <typename T, int R>
+__device__ __forceinline__ void to_lds(unsigned char *lds, const cplx<T> (&a)[1 << R], unsigned st,
                                        const unsigned *__restrict__ sj) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < (1 << R); ++j) {
         amp_vec<T> v;
         v.x = a[j].re;
         v.y = a[j].im;
@@ -335,11 +358,11 @@ __device__ __forceinline__ void to_lds(unsigned char *lds, const cplx<T> (&a)[16
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void from_lds(cplx<T> (&a)[16], const unsigned char *lds, unsigned st,
+template <typename T, int R>
+__device__ __forceinline__ void from_lds(cplx<T> (&a)[1 << R], const unsigned char *lds, unsigned st,
                                          const unsigned *__restrict__ sj) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < (1 << R); ++j) {
         const amp_vec<T> v = *reinterpret_cast<const amp_vec<T> *>(lds + (st ^ sj[j]));
         a[j] = cplx<T>{v.x, v.y};
     }
@@ -347,10 +370,10 @@ __device__ __forceinline__ void from_lds(cplx<T> (&a)[16], const unsigned char *
 
 // Diagonal with one register bit (slot P): registers with that bit set take
 // d1, the others d0, chosen at compile time.
-template <typename T, int P>
-__device__ __forceinline__ void reg_diag_split(cplx<T> (&a)[16], cplx<T> d0, cplx<T> d1) {
+template <typename T, int P, int R>
+__device__ __forceinline__ void reg_diag_split(cplx<T> (&a)[1 << R], cplx<T> d0, cplx<T> d1) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) a[j] = cmul(((j >> P) & 1) ? d1 : d0, a[j]);
+    for (int j = 0; j < (1 << R); ++j) a[j] = cmul(((j >> P) & 1) ? d1 : d0, a[j]);
 }
 
 // Fused 5-qubit unitary on FP32 MFMA (complex64 tiles).  The tile, in LDS,
@@ -420,7 +443,7 @@ __device__ __forceinline__ void fused5_mfma(cplx<float> *lds, const cplx<float> 
 // the exact one (0); FAST = 2 is the complex64 contracted instantiation that
 // also carries the MFMA block op (T_FUSED5), kept apart because its
 // accumulators cost registers (occupancy) in sweeps that have no blocks.
-template <typename T, int B, int FAST>
+template <typename T, int B, int FAST, int R>
 __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const TilePass &pd,
                                            const TileOp *__restrict__ ops, const cplx<T> *__restrict__ mats,
                                            uint64_t ntiles, double *__restrict__ partials) {
@@ -431,18 +454,19 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
     const unsigned *laytab =
         reinterpret_cast<const unsigned *>(reinterpret_cast<const unsigned char *>(mats) + pd.lay_off);
     const unsigned t = threadIdx.x;
-    Layout<T> L;
+    constexpr int NA = 1 << R;  // amplitudes per thread
+    Layout<T, R> L;
     // global offsets of the load/store layouts: per-thread part + 4 uniform
     // per-register-bit offsets (additive over disjoint bits)
-    uint64_t g_t, gs_t, g_r[4], gs_r[4];
+    uint64_t g_t, gs_t, g_r[R], gs_r[R];
     {
-        Layout<T> L0;
-        make_layout<T, B>(L0, pd.load_rb, t);
+        Layout<T, R> L0;
+        make_layout<T, B, R>(L0, pd.load_rb, t);
         g_t = gmap<B>(L0.tpart, pd);
-        make_layout<T, B>(L0, pd.store_rb, t);
+        make_layout<T, B, R>(L0, pd.store_rb, t);
         gs_t = gmap<B>(L0.tpart, pd);
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm) {
+        for (int mm = 0; mm < R; ++mm) {
             g_r[mm] = uint64_t(1) << pd.tq[(pd.load_rb >> (4 * mm)) & 15];
             gs_r[mm] = uint64_t(1) << pd.tq[(pd.store_rb >> (4 * mm)) & 15];
         }
@@ -460,16 +484,16 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
         if (!(((uint64_t(gridDim.x) - 1) >> j) & 1)) hmask |= qb;
     }
     auto next_high = [&](uint64_t h) { return ((h | ~hmask) + 1) & hmask; };
-    cplx<T> nxt[16];
+    cplx<T> nxt[NA];
     uint64_t tile = blockIdx.x;
     uint64_t high = 0;  // deposit of the current tile's high index bits
     if (tile < ntiles) {
         const uint64_t base = dep_lo + g_t;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
+        for (int j = 0; j < NA; ++j) {
             uint64_t off = base;
 #pragma unroll
-            for (int mm = 0; mm < 4; ++mm)
+            for (int mm = 0; mm < R; ++mm)
                 if ((j >> mm) & 1) off += g_r[mm];
             nxt[j] = amps[off];
         }
@@ -481,24 +505,24 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
     // for the loads then leaves the stores in flight.
     auto process = [&](uint64_t tile) __attribute__((always_inline)) {
         const uint64_t base = dep_lo | high;
-        cplx<T> a[16];
+        cplx<T> a[NA];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) a[j] = nxt[j];
+        for (int j = 0; j < NA; ++j) a[j] = nxt[j];
         const uint64_t next = tile + gridDim.x;
         if (next < ntiles) {
             const uint64_t nb = (dep_lo | next_high(high)) + g_t;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
+            for (int j = 0; j < NA; ++j) {
                 uint64_t off = nb;
 #pragma unroll
-                for (int mm = 0; mm < 4; ++mm)
+                for (int mm = 0; mm < R; ++mm)
                     if ((j >> mm) & 1) off += g_r[mm];
                 nxt[j] = amps[off];
             }
         }
-        make_layout<T, B>(L, pd.load_rb, t);
-        unsigned st = lds_base<T, B>(L);
-        const unsigned *cs = laytab + 16 * pd.load_lay;
+        make_layout<T, B, R>(L, pd.load_rb, t);
+        unsigned st = lds_base<T, B, R>(L);
+        const unsigned *cs = laytab + NA * pd.load_lay;
         int cur = pd.load_rb;
         OpHead h = op_head(ops + pd.op_begin);
         for (int o = 0; o < pd.op_count; ++o) {
@@ -507,12 +531,12 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
             const OpHead hn = op_head(ops + pd.op_begin + (o + 1 < pd.op_count ? o + 1 : o));
             if (h.relayout) {
                 __syncthreads();
-                to_lds<T>(smem_raw, a, st, cs);
+                to_lds<T, R>(smem_raw, a, st, cs);
                 __syncthreads();
-                make_layout<T, B>(L, h.rb, t);
-                st = lds_base<T, B>(L);
-                cs = laytab + 16 * h.lay;
-                from_lds<T>(a, smem_raw, st, cs);
+                make_layout<T, B, R>(L, h.rb, t);
+                st = lds_base<T, B, R>(L);
+                cs = laytab + NA * h.lay;
+                from_lds<T, R>(a, smem_raw, st, cs);
                 cur = h.rb;
             }
             const cplx<T> *m = mats + h.mat;
@@ -525,14 +549,14 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
             }
             switch (h.kind) {
             case T_DENSE1:
-                dispatch_dense1<T, FAST>(a, h.p, h.variant, m);
+                dispatch_dense1<T, FAST, R>(a, h.p, h.variant, m);
                 break;
             case T_CTRL1:
                 if ((unsigned(tile) & op.ocmask) == op.ocval)
-                    dispatch_ctrl1<T, FAST>(a, h.p, h.variant, m, L, op.cmask, op.cval);
+                    dispatch_ctrl1<T, FAST, R>(a, h.p, h.variant, m, L, op.cmask, op.cval);
                 break;
             case T_DENSE2:
-                dispatch_dense2<T, FAST>(a, h.p, h.q, h.variant, m);
+                dispatch_dense2<T, FAST, R>(a, h.p, h.q, h.variant, m);
                 break;
             case T_DIAGT: {
                 // e = thread part | register part; the thread part picks the
@@ -550,22 +574,27 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
                 if (h.p < 0) {
                     const cplx<T> d = pick4(et, m0, m1, m2, m3);
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) a[j] = cmul(d, a[j]);
+                    for (int j = 0; j < NA; ++j) a[j] = cmul(d, a[j]);
                 } else {
                     const int rw = 1 << (op.k - 1 - h.q);  // weight of the register bit in e
                     const cplx<T> d0 = pick4(et, m0, m1, m2, m3), d1 = pick4(et | rw, m0, m1, m2, m3);
                     switch (h.p) {
-                    case 0: reg_diag_split<T, 0>(a, d0, d1); break;
-                    case 1: reg_diag_split<T, 1>(a, d0, d1); break;
-                    case 2: reg_diag_split<T, 2>(a, d0, d1); break;
-                    default: reg_diag_split<T, 3>(a, d0, d1); break;
+                    case 0: reg_diag_split<T, 0, R>(a, d0, d1); break;
+                    case 1: reg_diag_split<T, 1, R>(a, d0, d1); break;
+                    case 2:
+                        if constexpr (R > 3) {
+                            reg_diag_split<T, 2, R>(a, d0, d1);
+                            break;
+                        }
+                        [[fallthrough]];
+                    default: reg_diag_split<T, R - 1, R>(a, d0, d1); break;
                     }
                 }
                 break;
             }
             case T_DIAG:
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
+                for (int j = 0; j < NA; ++j) {
                     const unsigned i = L.tpart | L.jp(j);
                     int e = 0;
                     for (int b2 = 0; b2 < op.k; ++b2) {
@@ -582,7 +611,7 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
                 const bool outer = bp >= kOuterBit;
                 const int ov = outer ? int((tile >> (bp - kOuterBit)) & 1) : 0;
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
+                for (int j = 0; j < NA; ++j) {
                     const unsigned i = L.tpart | L.jp(j);
                     const int v = outer ? ov : int((i >> (bp & 31)) & 1);
                     if (v == op.outcome) {
@@ -595,26 +624,26 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
                 break;
             }
             case T_FUSED5:
-                if constexpr (sizeof(T) == 4 && B >= 12 && FAST == 2) {  // only the MFMA instantiation
+                if constexpr (sizeof(T) == 4 && B >= 12 && FAST == 2 && R == 4) {  // only the MFMA instantiation
                     __syncthreads();
-                    to_lds<T>(smem_raw, a, st, cs);
+                    to_lds<T, R>(smem_raw, a, st, cs);
                     __syncthreads();
                     fused5_mfma<B>(reinterpret_cast<cplx<float> *>(lds), reinterpret_cast<const cplx<float> *>(m),
                                    op, t);
                     __syncthreads();
-                    from_lds<T>(a, smem_raw, st, cs);
+                    from_lds<T, R>(a, smem_raw, st, cs);
                 }
                 break;
             default: {  // T_LDSOP
                 __syncthreads();
-                to_lds<T>(smem_raw, a, st, cs);
+                to_lds<T, R>(smem_raw, a, st, cs);
                 __syncthreads();
                 const int k = op.k;
                 const int dim = 1 << k;
                 unsigned gm = 0;
                 for (int b2 = 0; b2 < k; ++b2) gm |= 1u << op.bit[b2];
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
+                for (int j = 0; j < NA; ++j) {
                     const unsigned i = L.tpart | L.jp(j);
                     int r = 0;
                     for (int b2 = 0; b2 < k; ++b2) r = (r << 1) | int((i >> op.bit[b2]) & 1);
@@ -636,17 +665,17 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
         }
         if (cur != pd.store_rb) {
             __syncthreads();
-            to_lds<T>(smem_raw, a, st, cs);
+            to_lds<T, R>(smem_raw, a, st, cs);
             __syncthreads();
-            make_layout<T, B>(L, pd.store_rb, t);
-            from_lds<T>(a, smem_raw, lds_base<T, B>(L), laytab + 16 * pd.store_lay);
+            make_layout<T, B, R>(L, pd.store_rb, t);
+            from_lds<T, R>(a, smem_raw, lds_base<T, B, R>(L), laytab + NA * pd.store_lay);
         }
         if (pd.red_bit >= 0) {
             const int rb2 = pd.red_bit;
             const bool outer = rb2 >= kOuterBit;
             const bool ov = outer && ((tile >> (rb2 - kOuterBit)) & 1);
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
+            for (int j = 0; j < NA; ++j) {
                 const bool on = outer ? ov : (((L.tpart | L.jp(j)) >> (rb2 & 31)) & 1);
                 const double re = double(a[j].re), im = double(a[j].im);
                 if (on) red += re * re + im * im;
@@ -654,10 +683,10 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
         }
         const uint64_t sb = base + gs_t;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
+        for (int j = 0; j < NA; ++j) {
             uint64_t off = sb;
 #pragma unroll
-            for (int mm = 0; mm < 4; ++mm)
+            for (int mm = 0; mm < R; ++mm)
                 if ((j >> mm) & 1) off += gs_r[mm];
             amps[off] = a[j];
         }
@@ -674,7 +703,7 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
         // fixed-shape block sum: butterflies within each group of kGroup
         // lanes (a wave, or the whole block when it is smaller than a wave:
         // 32 threads at B = 9), then the group totals in order
-        constexpr int kThr = 1 << (B - 4);
+        constexpr int kThr = 1 << (B - R);
         constexpr int kGroup = kThr < 64 ? kThr : 64;
         for (int off = kGroup / 2; off > 0; off >>= 1) red += __shfl_xor(red, off, kGroup);
         __syncthreads();
@@ -694,7 +723,20 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                                                            const TileOp *__restrict__ ops,
                                                            const cplx<T> *__restrict__ mats, uint64_t ntiles,
                                                            double *__restrict__ partials) {
-    tile_sweep<T, B, FAST>(amps, pd, ops, mats, ntiles, partials);
+    tile_sweep<T, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials);
+}
+
+// Eight amplitudes per thread (3 register bits): twice the threads of
+// k_tile_reg for the same tile and LDS, half the registers per thread, so a
+// 12-qubit complex128 tile runs 8 waves per workgroup and two workgroups per
+// CU give 4 waves per SIMD (k_tile_reg: 2) to hide FP64, LDS and barrier
+// latencies.  Ops on 2 register bits (T_DENSE2) need 2 of the 3 bits.
+template <typename T, int B, int FAST>
+__global__ __launch_bounds__(1 << (B - 3), 4) void k_tile_reg8(cplx<T> *__restrict__ amps, TilePass pd,
+                                                               const TileOp *__restrict__ ops,
+                                                               const cplx<T> *__restrict__ mats, uint64_t ntiles,
+                                                               double *__restrict__ partials) {
+    tile_sweep<T, B, FAST, 3>(amps, pd, ops, mats, ntiles, partials);
 }
 
 // complex64 without the MFMA block op (FAST = 1): the sweep fits 128 VGPRs,
@@ -706,7 +748,7 @@ __global__ __launch_bounds__(1 << (B - 4), 4) void k_tile_reg_c64(cplx<float> *_
                                                                   const TileOp *__restrict__ ops,
                                                                   const cplx<float> *__restrict__ mats,
                                                                   uint64_t ntiles, double *__restrict__ partials) {
-    tile_sweep<float, B, FAST>(amps, pd, ops, mats, ntiles, partials);
+    tile_sweep<float, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials);
 }
 
 }  // namespace dq

@@ -21,7 +21,32 @@ hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass 
     return hipGetLastError();
 }
 
+template <int B>
+hipError_t launch8(unsigned grid, hipStream_t stream, void *amps, const TilePass &pd, const TileOp *ops,
+                   const void *mats, uint64_t ntiles, double *partials) {
+    using T = double;
+    const size_t shmem = sizeof(cplx<T>) << B;
+    if (shmem > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg8<T, B, 1>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_tile_reg8<T, B, 1>), dim3(grid), dim3(1 << (B - 3)), shmem, stream,
+                       static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles, partials);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t dq_launch_tile_reg8_f64_fma(int b, unsigned grid, hipStream_t stream, void *amps, const TilePass &pd,
+                                       const TileOp *ops, const void *mats, uint64_t ntiles, double *partials) {
+    switch (b) {
+    case 9: return launch8<9>(grid, stream, amps, pd, ops, mats, ntiles, partials);
+    case 12: return launch8<12>(grid, stream, amps, pd, ops, mats, ntiles, partials);
+    case 13: return launch8<13>(grid, stream, amps, pd, ops, mats, ntiles, partials);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t dq_launch_tile_reg_f64_fma(int b, unsigned grid, hipStream_t stream, void *amps, const TilePass &pd,
                                    const TileOp *ops, const void *mats, uint64_t ntiles, double *partials) {

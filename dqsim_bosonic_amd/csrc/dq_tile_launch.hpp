@@ -13,6 +13,9 @@ hipError_t dq_launch_tile_reg_f64(int b, unsigned grid, hipStream_t stream, void
                                   double *partials);
 hipError_t dq_launch_tile_reg_f64_fma(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
                                       const dq::TileOp *ops, const void *mats, uint64_t ntiles, double *partials);
+// Eight amplitudes per thread (k_tile_reg8): layouts of 3 register bits.
+hipError_t dq_launch_tile_reg8_f64_fma(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
+                                       const dq::TileOp *ops, const void *mats, uint64_t ntiles, double *partials);
 hipError_t dq_launch_tile_reg_f32(int b, unsigned grid, hipStream_t stream, void *amps, const dq::TilePass &pd,
                                   const dq::TileOp *ops, const void *mats, uint64_t ntiles,
                                   double *partials);

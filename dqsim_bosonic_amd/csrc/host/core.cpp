@@ -279,6 +279,7 @@ PYBIND11_MODULE(_core, m) {
         .def("fused_stats", &Engine::fused_stats)
         .def("set_fuse_min_cost", &Engine::set_fuse_min_cost)
         .def("set_pair_fusion", &Engine::set_pair_fusion)
+        .def("set_reg_bits", &Engine::set_reg_bits)
         .def("compare", &Engine::compare)
         .def("p1_record", &Engine::p1_record)
         .def("result", &Engine::result);

@@ -185,6 +185,7 @@ class Engine {
     }
     void set_fuse_min_cost(double c) { check(dq_set_fuse_min_cost(st_->s, c), "fusion cost threshold"); }
     void set_pair_fusion(bool on) { check(dq_set_pair_fusion(st_->s, on ? 1 : 0), "pair fusion"); }
+    void set_reg_bits(int r) { check(dq_set_reg_bits(st_->s, r), "register bits"); }
     py::tuple fused_stats() {
         uint64_t a = 0, b = 0;
         check(dq_get_fused_stats(st_->s, &a, &b), "fused stats");

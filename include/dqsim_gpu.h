@@ -187,6 +187,13 @@ int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches,
  * into one op (their FP64 product, classified again). */
 int dq_set_pair_fusion(dq_state *s, int on);
 
+/* Register layout of the complex128 sweeps at fusion levels 2-3 (no
+ * reference counterpart; tuning only): 4 (default) keeps 16 amplitudes per
+ * thread in registers (k_tile_reg, 2 waves per SIMD), 3 keeps 8
+ * (k_tile_reg8, 4 waves per SIMD, more layout changes).  Results agree to
+ * rounding. */
+int dq_set_reg_bits(dq_state *s, int reg_bits);
+
 /* Fusion level 3 cost threshold (no reference counterpart; tuning only): a
  * block of gates is fused into one MFMA GEMM only when its gates' estimated
  * cost is at least min_cost FP32 flops per amplitude (default 128, about

@@ -124,3 +124,28 @@ def test_c64_fusion3_random_blocks_16q(min_cost):
     assert {int(k): v for k, v in r64.items()} == ref.classical_bits
     # p1 sums in a different order than the oracle's, so collapses may differ in the last bits
     assert np.max(np.abs(e64.result().statevector - ref.statevector)) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [12, 14, 20, 27])
+def test_eight_amplitude_layouts_equal_sixteen(n):
+    """The complex128 sweep with 3 register bits per layout (k_tile_reg8, 8
+    amplitudes per thread) against the default 4 (k_tile_reg): the same ops
+    on the same tiles in other register layouts, so records are equal and
+    amplitudes agree to rounding (1e-12); at 12 and 14 qubits both also
+    equal the oracle to 1e-12.  27 qubits: several tiles per workgroup."""
+    circ = W.random_blocks(n, 12, seed=n, remote_per_kind=4, measure_every=4)
+    text = circ.model_dump_json()
+    engines = []
+    for r in (3, 4):
+        e = _core.Engine(n, 2, 0, 0, 0)
+        e.set_reg_bits(r)
+        e.load(text)
+        engines.append((e, e.step(17)))
+    (e3, r3), (e4, r4) = engines
+    assert r3 == r4
+    d = e3.compare(e4)
+    assert d["max_abs"] <= 1e-12
+    if n <= 14:
+        ref = OracleSimulator(seed=17).simulate(circ)
+        assert {int(k): v for k, v in r3.items()} == ref.classical_bits
+        assert np.max(np.abs(e3.result().statevector - ref.statevector)) <= 1e-12
