@@ -297,11 +297,21 @@ __global__ __launch_bounds__(kPassThreads) void k_adjoint_pass(cplx<T> *__restri
 // they need no barrier.  A relayout step moves the tile through LDS to a new
 // set of register bits; it is the only step with barriers.
 
-enum AdjRKind : int { R_DENSE1 = 0, R_DENSE2 = 1, R_DIAG = 2, R_GRAD = 3, R_RELAYOUT = 4, R_DTABLE = 5 };
+enum AdjRKind : int {
+    R_DENSE1 = 0,
+    R_DENSE2 = 1,
+    R_DIAG = 2,
+    R_GRAD = 3,
+    R_RELAYOUT = 4,
+    R_DTABLE = 5,
+    R_DENSE1R = 6,  // 2x2 with real entries on register slot a
+    R_GRAD1 = 7     // one term, X (b = 0) or Y (b = 1) on register slot a alone
+};
 
 struct AdjRStep {
     int kind;
-    int a, b;          // DENSE1: register slot a; DENSE2: slots a (matrix MSB), b; GRAD: a = 1 if all terms are Z strings
+    int a, b;          // DENSE1(R): register slot a; DENSE2: slots a (matrix MSB), b; GRAD: a = 1 if all terms
+                       // are Z strings; GRAD1: slot a, b = 1 for Y
     int k;             // DIAG: 1 or 2
     int bits[2];       // DIAG: tile bits, MSB first
     int mat;           // matrix offset (complex entries)
@@ -355,6 +365,44 @@ __device__ __forceinline__ void r_dense1(cplx<T> (&a)[NR], const cplx<T> u0, con
         const cplx<T> x0 = a[j], x1 = a[j1];
         a[j] = cadd(cmul(u0, x0), cmul(u1, x1));
         a[j1] = cadd(cmul(u2, x0), cmul(u3, x1));
+    }
+}
+
+// 2x2 with real entries: each output is two real-times-complex products.
+template <typename T, int NR, int M>
+__device__ __forceinline__ void r_dense1r(cplx<T> (&a)[NR], const T u0, const T u1, const T u2, const T u3) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        if (j & (1 << M)) continue;
+        const int j1 = j | (1 << M);
+        const cplx<T> x0 = a[j], x1 = a[j1];
+        a[j] = cplx<T>{u0 * x0.re + u1 * x1.re, u0 * x0.im + u1 * x1.im};
+        a[j1] = cplx<T>{u2 * x0.re + u3 * x1.re, u2 * x0.im + u3 * x1.im};
+    }
+}
+
+// <lam| P |psi> summed over this thread's pairs for P = X (Y = 0) or Y (Y = 1)
+// on register slot M: with s0 = -1 for Y (the Z factor on the partner of a
+// bit-0 amplitude) and +1 for X, the sum is sum over pairs of
+// s0 conj(l0) p1 + conj(l1) p0 (the term's i^{#Y} is in its coefficient).
+// Returns (Re, Im) of that sum.
+template <typename T, int NR, int M, int Y>
+__device__ __forceinline__ void r_pair_inner(const cplx<T> (&p)[NR], const cplx<T> (&l)[NR], double &ar, double &ai) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        if (j & (1 << M)) continue;
+        const int j1 = j | (1 << M);
+        const double l0r = double(l[j].re), l0i = double(l[j].im), l1r = double(l[j1].re), l1i = double(l[j1].im);
+        const double p0r = double(p[j].re), p0i = double(p[j].im), p1r = double(p[j1].re), p1i = double(p[j1].im);
+        ar += l1r * p0r + l1i * p0i;
+        ai += l1r * p0i - l1i * p0r;
+        if (Y) {
+            ar -= l0r * p1r + l0i * p1i;
+            ai -= l0r * p1i - l0i * p1r;
+        } else {
+            ar += l0r * p1r + l0i * p1i;
+            ai += l0r * p1i - l0i * p1r;
+        }
     }
 }
 
@@ -548,6 +596,33 @@ __global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoi
                 }
                 break;
             }
+            case R_DENSE1R: {
+                const T u0 = T(mats[st.mat].re), u1 = T(mats[st.mat + 1].re);
+                const T u2 = T(mats[st.mat + 2].re), u3 = T(mats[st.mat + 3].re);
+                switch (st.a) {
+                case 0: r_dense1r<T, NR, 0>(p, u0, u1, u2, u3); r_dense1r<T, NR, 0>(l, u0, u1, u2, u3); break;
+                case 1: r_dense1r<T, NR, 1 % RB>(p, u0, u1, u2, u3); r_dense1r<T, NR, 1 % RB>(l, u0, u1, u2, u3); break;
+                default: r_dense1r<T, NR, 2 % RB>(p, u0, u1, u2, u3); r_dense1r<T, NR, 2 % RB>(l, u0, u1, u2, u3); break;
+                }
+                break;
+            }
+            case R_GRAD1: {
+                double ar = 0.0, ai = 0.0;
+                const int code = st.a * 2 + st.b;
+                switch (code) {
+                case 0: r_pair_inner<T, NR, 0, 0>(p, l, ar, ai); break;
+                case 1: r_pair_inner<T, NR, 0, 1>(p, l, ar, ai); break;
+                case 2: r_pair_inner<T, NR, 1 % RB, 0>(p, l, ar, ai); break;
+                case 3: r_pair_inner<T, NR, 1 % RB, 1>(p, l, ar, ai); break;
+                case 4: r_pair_inner<T, NR, 2 % RB, 0>(p, l, ar, ai); break;
+                default: r_pair_inner<T, NR, 2 % RB, 1>(p, l, ar, ai); break;
+                }
+                const AdjRTerm tm = terms[st.term0];
+                double acc = tm.cre * ai + tm.cim * ar;  // Im(c (ar + i ai))
+                acc = wave_sum_lane63(acc);
+                if (lane == 63) partials[(uint64_t(st.slot) * ntiles + tile) * W + wave] = acc;
+                break;
+            }
             case R_DENSE2: {
                 cplx<T> u[16];
 #pragma unroll
@@ -659,6 +734,27 @@ __global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoi
             lam[g] = l[j];
         }
         __syncthreads();  // the next tile's first relayout writes the LDS
+    }
+}
+
+// First level of the gradient sums: block (s, c) adds chunk c (of kReduceChunks
+// contiguous chunks) of slot s's partials, in order.
+constexpr int kReduceChunks = 32;
+__global__ __launch_bounds__(kThreads) void k_adjoint_reduce_chunks(const double *__restrict__ partials,
+                                                                    uint64_t count, double *__restrict__ chunks) {
+    __shared__ double scratch[kThreads / 64];
+    const int s = blockIdx.x, c = blockIdx.y;
+    const uint64_t per = (count + kReduceChunks - 1) / kReduceChunks;
+    const uint64_t lo = uint64_t(c) * per, hi = std::min<uint64_t>(count, lo + per);
+    double acc = 0.0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += kThreads) acc += partials[uint64_t(s) * count + i];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) r += scratch[w];
+        chunks[s * kReduceChunks + c] = r;
     }
 }
 
@@ -810,10 +906,14 @@ bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm>
         std::memset(&r, 0, sizeof r);
         r.mat = x.mat;
         switch (x.kind) {
-        case A_DENSE1:
+        case A_DENSE1: {
             r.kind = R_DENSE1;
             r.a = slot_of(mask, x.bit[0]);
+            bool real = true;
+            for (int e = 0; e < 4; ++e) real = real && hmats[size_t(x.mat + e)].im == 0.0;
+            if (real) r.kind = R_DENSE1R;
             break;
+        }
         case A_DENSE2:
             r.kind = R_DENSE2;
             r.a = slot_of(mask, x.bit[0]);
@@ -833,6 +933,15 @@ bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm>
             r.a = 1;  // 1: every term is a Z string
             for (int q = 0; q < x.nterms; ++q)
                 if (hterms[size_t(x.term0 + q)].x) r.a = 0;
+            if (x.nterms == 1) {
+                // one X or Y on a single qubit: the pair form
+                const AdjTerm &tm = hterms[size_t(x.term0)];
+                if (__builtin_popcount(tm.x) == 1 && (tm.z == 0 || tm.z == tm.x)) {
+                    r.kind = R_GRAD1;
+                    r.a = slot_of(mask, __builtin_ctz(tm.x));
+                    r.b = tm.z ? 1 : 0;
+                }
+            }
             for (int q = 0; q < x.nterms; ++q) {
                 const AdjTerm &tm = hterms[size_t(x.term0 + q)];
                 AdjRTerm rt;
@@ -1133,7 +1242,8 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
     const size_t b_tables = b_mats + b_steps + b_terms + b_rsteps + b_rterms;
     const size_t b_part = up(max_slots * ntiles * 4 * sizeof(double));
     const size_t b_grads = up(slots_total * sizeof(double));
-    const size_t b_total = b_tables + b_part + b_grads;
+    const size_t b_chunks = up(max_slots * kReduceChunks * sizeof(double));
+    const size_t b_total = b_tables + b_part + b_grads + b_chunks;
     if (psi->d_adj_cap < b_total) {
         if (psi->d_adj) (void)hipFree(psi->d_adj);
         psi->d_adj = nullptr;
@@ -1163,6 +1273,7 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
     void *d_rterms = dbase + b_mats + b_steps + b_terms + b_rsteps;
     double *d_part = reinterpret_cast<double *>(dbase + b_tables);
     double *d_grads = reinterpret_cast<double *>(dbase + b_tables + b_part);
+    double *d_chunks = reinterpret_cast<double *>(dbase + b_tables + b_part + b_grads);
     {
         unsigned char *h = psi->h_adj;
         std::memcpy(h, hmats.data(), hmats.size() * sizeof(cplx<double>));
@@ -1205,8 +1316,11 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
             ADJ_TRY(e);
             const size_t ns = pass_grad_ops[p].size();
             if (ns) {
-                hipLaunchKernelGGL(k_adjoint_reduce, dim3(unsigned(ns)), dim3(kThreads), 0, stream, d_part, count,
-                                   d_grads + slot_base);
+                hipLaunchKernelGGL(k_adjoint_reduce_chunks, dim3(unsigned(ns), kReduceChunks), dim3(kThreads), 0,
+                                   stream, d_part, count, d_chunks);
+                ADJ_TRY(hipGetLastError());
+                hipLaunchKernelGGL(k_adjoint_reduce, dim3(unsigned(ns)), dim3(kThreads), 0, stream, d_chunks,
+                                   uint64_t(kReduceChunks), d_grads + slot_base);
                 ADJ_TRY(hipGetLastError());
             }
             slot_base += ns;
