@@ -212,3 +212,22 @@ def test_tiled_reverse_sweep_with_three_qubit_gates(n):
     e_p, g_p = est.gradient(th)
     assert abs(e_t - e_p) <= 1e-13
     assert np.max(np.abs(g_t - g_p)) <= 1e-12
+
+
+def test_benchmark_ansatz_26q_against_fd_and_parameter_shift():
+    """The differentiation benchmark at its stated size: 26 qubits, 40
+    layers, 2080 parameters, H = ZZ chain + 0.7 X sum.  On 6 random
+    parameters of a random parameter set, the adjoint gradient equals the
+    two-term parameter-shift rule (1e-10) and central differences with
+    h = 1e-6 (1e-7: FD roundoff ~1e-16 |E| / h ~ 1e-9, truncation ~h^2)."""
+    pc, H = A.hea_benchmark(26, 40, 42)
+    assert pc.num_params == 2080
+    th = np.random.default_rng(9).uniform(0.0, 2 * math.pi, pc.num_params)
+    est = A.Estimator(pc, H)
+    e, g = est.gradient(th)
+    assert abs(e - est.expectation(th)) < 1e-11
+    idx = np.random.default_rng(10).choice(pc.num_params, size=6, replace=False)
+    ps = est.parameter_shift(th, indices=idx)
+    assert np.max(np.abs(g[idx] - ps)) <= 1e-10, np.max(np.abs(g[idx] - ps))
+    fd = est.finite_difference(th, indices=idx, h=1e-6)
+    assert np.max(np.abs(g[idx] - fd)) <= 1e-7, np.max(np.abs(g[idx] - fd))
