@@ -28,6 +28,7 @@ EXPORTS = (
     "dq_get_fused_stats", "dq_set_fuse_min_cost", "dq_set_pair_fusion", "dq_set_reg_bits", "dq_top_k",
     "dq_batch_create", "dq_batch_destroy", "dq_batch_add_instance", "dq_batch_apply",
     "dq_batch_set_observable", "dq_batch_set_workers", "dq_batch_run", "dq_batch_stats",
+    "dq_batch_set_autotune", "dq_batch_path_stats", "dq_batch_tuned",
     "dq_shots_create", "dq_shots_destroy", "dq_shots_add_gate", "dq_shots_add_measure", "dq_shots_add_reset",
     "dq_shots_add_conditional", "dq_shots_words", "dq_shots_run",
     "dq_mps_create", "dq_mps_destroy", "dq_mps_load", "dq_mps_to_state", "dq_mps_prob_one", "dq_mps_set_cut",
@@ -79,6 +80,10 @@ def lib() -> ctypes.CDLL:
         L.dq_batch_set_workers.argtypes = [vp, ctypes.c_int]
         L.dq_batch_run.argtypes = [vp, _dp]
         L.dq_batch_stats.argtypes = [vp, u64p, u64p, _dp]
+        L.dq_batch_set_autotune.argtypes = [vp, ctypes.c_int]
+        L.dq_batch_path_stats.argtypes = [vp, u64p, _dp, _dp, _dp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.dq_batch_tuned.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, ip]
         L.dq_shots_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
         L.dq_shots_destroy.argtypes = [vp]
         L.dq_shots_destroy.restype = None
@@ -306,6 +311,29 @@ class Batch:
         a, b, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
         _check(lib().dq_batch_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
         return a.value, b.value, t.value
+
+    def set_autotune(self, on: bool) -> None:
+        _check(lib().dq_batch_set_autotune(self.h, int(bool(on))))
+
+    def path_stats(self) -> dict:
+        """Instances per path (lds, persistent, engine), graph GPU ms, engine
+        wall seconds and algorithmic bytes per path of the last run."""
+        cnt = (ctypes.c_uint64 * 3)()
+        by = (ctypes.c_double * 3)()
+        g, e = ctypes.c_double(), ctypes.c_double()
+        _check(lib().dq_batch_path_stats(self.h, cnt, ctypes.byref(g), ctypes.byref(e), by))
+        return {"lds": cnt[0], "persistent": cnt[1], "engine": cnt[2], "graph_ms": g.value, "engine_s": e.value,
+                "bytes": [by[0], by[1], by[2]]}
+
+
+def batch_tuned(n: int, dtype: int = C128):
+    """The batch autotuner's cached choice for (n, dtype), or None."""
+    v = [ctypes.c_int() for _ in range(4)]
+    rc = lib().dq_batch_tuned(n, dtype, *[ctypes.byref(x) for x in v])
+    if rc != 0:
+        return None
+    return {"path": "persistent" if v[0].value == 1 else "engine", "fusion": v[1].value, "tile_bits": v[2].value,
+            "reg_bits": v[3].value}
 
 
 class Shots:

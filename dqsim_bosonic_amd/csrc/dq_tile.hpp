@@ -443,10 +443,15 @@ __device__ __forceinline__ void fused5_mfma(cplx<float> *lds, const cplx<float> 
 // the exact one (0); FAST = 2 is the complex64 contracted instantiation that
 // also carries the MFMA block op (T_FUSED5), kept apart because its
 // accumulators cost registers (occupancy) in sweeps that have no blocks.
+//
+// The sweep walks tiles vblock, vblock + vgrid, ... (vgrid a power of two):
+// a launch passes blockIdx.x / gridDim.x; a workgroup that owns a whole state
+// passes 0 / 1 and walks every tile in order.
 template <typename T, int B, int FAST, int R>
 __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const TilePass &pd,
                                            const TileOp *__restrict__ ops, const cplx<T> *__restrict__ mats,
-                                           uint64_t ntiles, double *__restrict__ partials) {
+                                           uint64_t ntiles, double *__restrict__ partials, uint64_t vblock,
+                                           uint64_t vgrid) {
     double red = 0.0;  // this thread's share of the epilogue reduction
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -472,20 +477,20 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
         }
     }
     // Global offset of a tile: its outer-index bits deposited on the outer
-    // qubits (oq ascending).  Tiles are blockIdx.x + k * gridDim.x with
-    // gridDim.x a power of two (the host launches it so), so the offset is
-    // the deposit of blockIdx.x, fixed per workgroup, ORed with the deposit
+    // qubits (oq ascending).  Tiles are vblock + k * vgrid with vgrid a
+    // power of two (the host launches it so), so the offset is the deposit
+    // of vblock, fixed per workgroup, ORed with the deposit
     // of k on the outer qubits of the high index bits (hmask); that one steps
     // from k to k + 1 as a counter confined to the bits of hmask.
     uint64_t dep_lo = 0, hmask = 0;
     for (int j = 0; j < pd.n_outer; ++j) {
         const uint64_t qb = uint64_t(1) << pd.oq[j];
-        if ((uint64_t(blockIdx.x) >> j) & 1) dep_lo |= qb;
-        if (!(((uint64_t(gridDim.x) - 1) >> j) & 1)) hmask |= qb;
+        if ((vblock >> j) & 1) dep_lo |= qb;
+        if (!(((vgrid - 1) >> j) & 1)) hmask |= qb;
     }
     auto next_high = [&](uint64_t h) { return ((h | ~hmask) + 1) & hmask; };
     cplx<T> nxt[NA];
-    uint64_t tile = blockIdx.x;
+    uint64_t tile = vblock;
     uint64_t high = 0;  // deposit of the current tile's high index bits
     if (tile < ntiles) {
         const uint64_t base = dep_lo + g_t;
@@ -508,7 +513,7 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
         cplx<T> a[NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) a[j] = nxt[j];
-        const uint64_t next = tile + gridDim.x;
+        const uint64_t next = tile + vgrid;
         if (next < ntiles) {
             const uint64_t nb = (dep_lo | next_high(high)) + g_t;
 #pragma unroll
@@ -694,7 +699,7 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
     };
     if (tile < ntiles) {
         process(tile);
-        for (tile += gridDim.x; tile < ntiles; tile += gridDim.x) {
+        for (tile += vgrid; tile < ntiles; tile += vgrid) {
             high = next_high(high);
             process(tile);
         }
@@ -713,7 +718,7 @@ __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const Til
         if (t == 0) {
             double r = 0.0;
             for (int w = 0; w < kThr / kGroup; ++w) r += scratch[w];
-            partials[blockIdx.x] = r;
+            partials[vblock] = r;
         }
     }
 }
@@ -723,7 +728,7 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                                                            const TileOp *__restrict__ ops,
                                                            const cplx<T> *__restrict__ mats, uint64_t ntiles,
                                                            double *__restrict__ partials) {
-    tile_sweep<T, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials);
+    tile_sweep<T, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x);
 }
 
 // Eight amplitudes per thread (3 register bits): twice the threads of
@@ -736,7 +741,7 @@ __global__ __launch_bounds__(1 << (B - 3), 4) void k_tile_reg8(cplx<T> *__restri
                                                                const TileOp *__restrict__ ops,
                                                                const cplx<T> *__restrict__ mats, uint64_t ntiles,
                                                                double *__restrict__ partials) {
-    tile_sweep<T, B, FAST, 3>(amps, pd, ops, mats, ntiles, partials);
+    tile_sweep<T, B, FAST, 3>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x);
 }
 
 // complex64 without the MFMA block op (FAST = 1): the sweep fits 128 VGPRs,
@@ -748,7 +753,7 @@ __global__ __launch_bounds__(1 << (B - 4), 4) void k_tile_reg_c64(cplx<float> *_
                                                                   const TileOp *__restrict__ ops,
                                                                   const cplx<float> *__restrict__ mats,
                                                                   uint64_t ntiles, double *__restrict__ partials) {
-    tile_sweep<float, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials);
+    tile_sweep<float, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x);
 }
 
 }  // namespace dq

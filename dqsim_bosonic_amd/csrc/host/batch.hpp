@@ -138,13 +138,32 @@ py::tuple simulate_hea_batch(py::array_t<int64_t, py::array::c_style | py::array
         rc = dq_batch_run(b, out.mutable_data());
     }
     check(rc, "simulate_hea_batch");
-    uint64_t lds = 0, eng = 0;
-    double ms = 0;
-    check(dq_batch_stats(b, &lds, &eng, &ms), "batch stats");
+    uint64_t cnt[3] = {0, 0, 0};
+    double graph_ms = 0, engine_s = 0, bytes[3] = {0, 0, 0};
+    check(dq_batch_path_stats(b, cnt, &graph_ms, &engine_s, bytes), "batch stats");
     py::dict st;
-    st["lds_instances"] = lds;
-    st["engine_instances"] = eng;
-    st["lds_gpu_ms"] = ms;
+    st["lds_instances"] = cnt[0];
+    st["persistent_instances"] = cnt[1];
+    st["engine_instances"] = cnt[2];
+    st["graph_gpu_ms"] = graph_ms;
+    st["engine_wall_s"] = engine_s;
+    st["persistent_bytes"] = bytes[1];
+    st["engine_bytes"] = bytes[2];
+    // the autotuner's choices for the widths of this batch
+    py::dict tuned;
+    std::set<int> widths;
+    for (size_t i = 0; i < count; ++i) widths.insert(int(ns.data()[i]));
+    for (int n : widths) {
+        int path = 0, fu = 0, tb = 0, rb = 0;
+        if (dq_batch_tuned(n, dt, &path, &fu, &tb, &rb) != DQ_OK) continue;
+        py::dict c;
+        c["path"] = path == 1 ? "persistent" : "engine";
+        c["fusion"] = fu;
+        c["tile_bits"] = tb;
+        c["reg_bits"] = rb;
+        tuned[py::int_(n)] = c;
+    }
+    st["tuned"] = tuned;
     return py::make_tuple(out[py::slice(0, py::ssize_t(count), 1)], st);
 }
 

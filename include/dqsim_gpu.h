@@ -212,10 +212,18 @@ int dq_get_fused_stats(const dq_state *s, uint64_t *blocks, uint64_t *gates);
  * unitary circuit from |0...0> on its own n qubits plus a diagonal
  * observable H = sum_t coeff_t Z(zmask_t) (default: the open chain
  * sum_{i=0}^{n-2} Z_i Z_{i+1}); dq_batch_run writes <psi|H|psi> of every
- * instance.  Instances whose state fits one workgroup's LDS (n <= 13 for
- * complex128, n <= 14 for complex64) run entirely in LDS, one workgroup per
- * instance; larger ones run on the statevector engine from `workers` host
- * threads, each with its own states and stream. */
+ * instance.  Three paths:
+ *   - LDS-resident: the state fits one workgroup's LDS (n <= 13 for
+ *     complex128, n <= 14 for complex64): one workgroup per instance;
+ *   - persistent: 14..20 qubits, one workgroup per instance in one launch
+ *     that takes instances from a device queue, state in a global arena;
+ *   - engine: larger instances on the statevector engine from `workers`
+ *     host threads, each with its own states and stream.
+ * The LDS and persistent work of a run is one captured hipGraph (upload,
+ * counter reset, launches, download).  Above the LDS limit the path and the
+ * engine's sweep configuration are chosen per (width, dtype) by an
+ * autotuner that times a four-layer proxy of the first instance of each
+ * width once per process and caches the choice. */
 typedef struct dq_batch dq_batch;
 
 int dq_batch_create(int dtype, dq_batch **out);
@@ -233,6 +241,19 @@ int dq_batch_run(dq_batch *b, double *expect);
 /* Instances on each path in the last run, and the LDS launches' GPU time. */
 int dq_batch_stats(const dq_batch *b, uint64_t *lds_instances, uint64_t *engine_instances,
                    double *lds_ms);
+/* Autotuning on (1, default) or off (0: widths up to 20 take the persistent
+ * path, larger ones the engine with its defaults). */
+int dq_batch_set_autotune(dq_batch *b, int on);
+/* Last run: counts[3] instances on the LDS / persistent / engine paths,
+ * *graph_ms the GPU time of the captured graph (LDS + persistent launches),
+ * *engine_s the wall time of the engine path, bytes[3] the algorithmic HBM
+ * bytes per path (one read and one write of the state per sweep, plus
+ * the initial write and the final read; LDS: 0). */
+int dq_batch_path_stats(const dq_batch *b, uint64_t *counts, double *graph_ms, double *engine_s, double *bytes);
+/* The autotuner's cached choice for (n_qubits, dtype): path 1 persistent or
+ * 2 engine, and the engine's fusion level, tile qubits and register bits.
+ * DQ_ERR_ARG if that width has not been tuned in this process. */
+int dq_batch_tuned(int n_qubits, int dtype, int *path, int *fusion, int *tile_bits, int *reg_bits);
 
 /* ---- batched shots ----
  * Replaces the per-shot loop of StatevectorSimulator::simulate_shots

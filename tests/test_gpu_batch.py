@@ -5,8 +5,10 @@ src/engine.rs (gate by gate from |0...0>), and <H> = sum_x |psi_x|^2 h(x)
 computed from it in numpy; the open ZZ chain is also cross-checked against
 probabilities() of the per-circuit simulator (a closed form over the
 distribution).  complex128 to 1e-12, complex64 to 1e-5 (FP32 rounding over
-a few hundred gates).  Instances cover both device paths: LDS-resident
-(n <= 13 complex128, n <= 14 complex64) and the engine workers (larger n).
+a few hundred gates).  Instances cover the three device paths: LDS-resident
+(n <= 13 complex128, n <= 14 complex64), the persistent launch (up to 20
+qubits) and the engine workers, the last two as the autotuner chooses, or
+forced with autotuning off.
 """
 from __future__ import annotations
 
@@ -94,7 +96,9 @@ def test_random_instances_both_paths(dtype, tol):
     got = b.run()
     lds, eng, _ = b.stats()
     lim = 13 if dtype == capi.C128 else 14
-    assert lds == sum(1 for n in widths if n <= lim) and eng == len(widths) - lds
+    ps = b.path_stats()
+    assert lds == sum(1 for n in widths if n <= lim) and eng == ps["engine"]
+    assert ps["persistent"] + ps["engine"] == len(widths) - lds
     assert np.max(np.abs(got - np.array(want))) <= tol, np.abs(got - np.array(want))
 
 
@@ -172,3 +176,95 @@ def test_hea_arrays_match_circuit_objects():
         assert np.max(np.abs(got - want)) <= 1e-14
         lim = 13 if dtype == "c128" else 14
         assert st["lds_instances"] == int(np.sum(ns <= lim))
+
+
+def hea_gates(n, d, angles):
+    ry = lambda t: orc.gate_matrix("ry", [t])  # noqa: E731
+    rz = lambda t: orc.gate_matrix("rz", [t])  # noqa: E731
+    cx = orc.gate_matrix("cx", [])
+    out = []
+    for layer in range(d):
+        for q in range(n):
+            out.append((ry(angles[layer][q][0]), [q]))
+        for q in range(n):
+            out.append((rz(angles[layer][q][1]), [q]))
+        for q in range(n):
+            out.append((cx, [q, (q + 1) % n]))
+    return out
+
+
+@pytest.mark.parametrize("dtype,tol", [(capi.C128, 1e-12), (capi.C64, 1e-5)])
+def test_persistent_path_random_instances(dtype, tol):
+    """Autotuning off: every instance of 14..20 qubits takes the persistent
+    launch (one workgroup per instance, tile passes from the engine's
+    planner, all in one captured graph).  Random gates of 1 to 5 qubits."""
+    rng = np.random.default_rng(21)
+    widths = [14, 15, 16, 17, 14, 15, 16, 14, 18, 15, 14, 16]
+    b = capi.Batch(dtype)
+    b.set_autotune(False)
+    insts = []
+    for n in widths:
+        gates = random_gates(n, 60, rng)
+        i = b.add(n)
+        for u, qs in gates:
+            b.apply(i, u, qs)
+        terms = zz_chain(n) if n % 2 else [(0.7, 1 << 3), (-1.3, (1 << 2) | (1 << (n - 1)))]
+        if n % 2 == 0:
+            b.set_observable(i, terms)
+        insts.append((n, gates, terms))
+    got = b.run()
+    st = b.path_stats()
+    lim = 13 if dtype == capi.C128 else 14  # complex64 14-qubit states still fit LDS
+    assert st["persistent"] == sum(1 for n in widths if n > lim) and st["engine"] == 0
+    assert st["lds"] == sum(1 for n in widths if n <= lim)
+    assert st["bytes"][1] > 0 and st["graph_ms"] > 0
+    for e, (n, gates, terms) in zip(got, insts):
+        assert abs(e - expect_z(oracle_state(n, gates), terms)) <= tol * max(1.0, abs(e))
+
+
+def test_persistent_path_288_mixed_hea_instances():
+    """288 hardware-efficient-ansatz instances of 10..17 qubits and 2..6
+    layers (LDS-resident up to 13 qubits, persistent above) in one run:
+    every <ZZ chain> against the oracle's statevector, 1e-12."""
+    ns, ds, angles = W.hea_batch_arrays(288, seed=8, n_range=(10, 17), d_range=(2, 6))
+    b = capi.Batch(capi.C128)
+    b.set_autotune(False)
+    ref = []
+    for n, d, a in zip(ns, ds, angles):
+        n, d = int(n), int(d)
+        gates = hea_gates(n, d, a)
+        i = b.add(n)
+        for u, qs in gates:
+            b.apply(i, u, qs)
+        ref.append((n, gates))
+    got = b.run()
+    st = b.path_stats()
+    assert st["persistent"] == int(np.sum(ns >= 14)) and st["lds"] == int(np.sum(ns <= 13))
+    err = max(abs(e - expect_z(oracle_state(n, g), zz_chain(n))) for e, (n, g) in zip(got, ref))
+    assert err <= 1e-12, err
+
+
+def test_autotuner_choice_and_results():
+    """With autotuning on, each width above the LDS limit gets a cached
+    choice (path, fusion level, tile qubits, register bits); the results
+    are the same whichever path it picks."""
+    rng = np.random.default_rng(3)
+    widths = [14, 15, 21]
+    b = capi.Batch(capi.C128)
+    insts = []
+    for n in widths:
+        gates = random_gates(n, 40, rng)
+        i = b.add(n)
+        for u, qs in gates:
+            b.apply(i, u, qs)
+        insts.append((n, gates))
+    got = b.run()
+    for n in widths:
+        t = capi.batch_tuned(n, capi.C128)
+        assert t is not None and t["path"] in ("persistent", "engine")
+        assert t["tile_bits"] in (12, 13) and t["reg_bits"] in (3, 4)
+    assert capi.batch_tuned(21, capi.C128)["path"] == "engine"
+    st = b.path_stats()
+    assert st["persistent"] + st["engine"] == 3
+    for e, (n, gates) in zip(got, insts):
+        assert abs(e - expect_z(oracle_state(n, gates), zz_chain(n))) <= 1e-12 * max(1.0, abs(e))
