@@ -461,3 +461,26 @@ def test_marginal_literal_out_of_range_and_repeated_qubits():
         b3 = (j >> 2) & 1 | j & 1
         b0 = (j >> 1) & 1
         assert abs(got[j] - p[(bit(3) == b3) & (bit(0) == b0)].sum()) < 1e-15
+
+
+def test_golden_30q_record_fixture_invariants():
+    """tests/golden/random_blocks_30q_seed42.json (written by the oracle's
+    OpenMP restatement): 8 measurements and 8 resets (16 p1 values in [0,
+    1]), the resets after a measurement of the same qubit see p1 = 0 or 1
+    exactly as that measurement's outcome, both 8-qubit marginals sum to 1
+    and the sampled amplitudes are normalised-size (|psi_i|^2 <= 1)."""
+    path = os.path.join(GOLDEN, "random_blocks_30q_seed42.json")
+    g = json.load(open(path))
+    assert g["circuit"] == {"generator": "workloads.random_blocks", "n": 30, "depth": 40, "seed": 42}
+    p1 = g["p1"]
+    assert len(p1) == 16 and all(-1e-12 <= p <= 1 + 1e-12 for p in p1)
+    bits = g["classical_bits"]
+    assert sorted(bits) == [str(k) for k in range(8)]
+    # per checkpoint: measure q1, measure q2, reset q1, reset q2
+    for c in range(4):
+        m1, m2, r1, r2 = p1[4 * c: 4 * c + 4]
+        for m, r, bit in ((m1, r1, bits[str(2 * c)]), (m2, r2, bits[str(2 * c + 1)])):
+            assert abs(r - float(bit)) <= 1e-12, (c, m, r, bit)
+    assert abs(sum(g["marg_low8"]) - 1.0) < 1e-12 and abs(sum(g["marg_high8"]) - 1.0) < 1e-12
+    amps = np.array(g["amp_re"]) + 1j * np.array(g["amp_im"])
+    assert amps.size == len(g["amp_index"]) == 1026 and np.all(np.abs(amps) <= 1.0)
