@@ -35,11 +35,16 @@ on <= 5 qubits multiplied into 32x32 blocks in FP64 and applied on FP32 MFMA
 inside the LDS-tiled sweep) and as complex128 (fusion level 2) on the same GPU
 (32 GiB + 64 GiB).  Reported: gates/s of each, and on device max |d amp|,
 1 - |<psi64|psi32>|^2 and the total variation distance of the probabilities.
+--precision-modes f2,f3,f3all,f3nopair,f3nopair_all also times the complex64
+step in other fusion modes (f2: VALU register ops only; f3: blocks that pass
+the cost threshold; f3all: every block; nopair: without pair fusion), which
+shows what moving gates from the VALU into MFMA blocks costs.
 
 batch (batching form): 4096 hardware-efficient-ansatz circuits, n ~ U{10..24},
 depth ~ U{8..64} (ry, rz on every qubit then a cx ring per layer), <sum_i
-Z_i Z_{i+1}> of each, complex128 and complex64: circuits/s, and the unfused
-algorithmic bytes (every gate reading and writing the whole state) per second.
+Z_i Z_{i+1}> of each, complex128 and complex64: circuits/s, the instances
+each path took (LDS-resident, persistent, engine), the widths' tuned
+configurations, and the state bytes the paths moved per second.
 
 mps (matrix-product states): Mps::to_statevector of a random 30-site MPS with
 bond <= 16 (upload, growth of both halves, tiled complex GEMM join writing the
@@ -252,7 +257,42 @@ def shots_leg(shots: int, seed: int, cpu_seconds: float) -> dict:
     return out
 
 
-def precision_leg(n: int, depth: int, seed: int, steps: int) -> dict:
+def c64_mode_engine(n: int, mode: str):
+    """A complex64 engine in one of the fusion modes of --precision-modes."""
+    from dqsim_bosonic_amd import _core
+
+    if mode not in ("f2", "f3", "f3all", "f3nopair", "f3nopair_all"):
+        raise ValueError(f"unknown complex64 fusion mode {mode!r}")
+    e = _core.Engine(n, 2 if mode == "f2" else 3, 0, 0, 1)
+    if mode.endswith("all"):
+        e.set_fuse_min_cost(0.0)
+    if "nopair" in mode:
+        e.set_pair_fusion(False)
+    return e
+
+
+def c64_modes(n: int, text: str, gates: int, seed: int, modes: list[str]) -> dict:
+    out = {}
+    for m in modes:
+        e = c64_mode_engine(n, m)
+        e.load(text)
+        e.step(seed)
+        e.set_timing(True)
+        f0, s0 = e.fused_stats(), e.stats()[0]
+        t0 = time.perf_counter()
+        e.step(seed)
+        dt = time.perf_counter() - t0
+        ms, cnt = e.timing()
+        f1, s1 = e.fused_stats(), e.stats()[0]
+        out[m] = {"s_per_step": round(dt, 3), "gates_per_s": round(gates / dt, 2), "sweeps": s1 - s0,
+                  "avg_sweep_ms": round(ms / max(cnt, 1), 3), "blocks": f1[0] - f0[0],
+                  "gates_in_blocks": f1[1] - f0[1]}
+        log(f"[bench] precision mode {m}: {out[m]}")
+        del e
+    return out
+
+
+def precision_leg(n: int, depth: int, seed: int, steps: int, modes: list[str] | None = None) -> dict:
     from dqsim_bosonic_amd import _core
     from dqsim_bosonic_amd import workloads as W
 
@@ -260,6 +300,8 @@ def precision_leg(n: int, depth: int, seed: int, steps: int) -> dict:
     gates = W.gate_count(circ)
     text = circ.model_dump_json()
     out = {"workload": f"random_blocks {n}q depth {depth} ({gates} gates), blocks 2x{n // 2}"}
+    if modes:
+        out["c64_modes"] = c64_modes(n, text, gates, seed, modes)
     e32 = _core.Engine(n, 3, 0, 0, 1)
     e32.load(text)
     rec32 = e32.step(seed)  # warm-up
@@ -395,6 +437,7 @@ def main() -> int:
     ap.add_argument("--fusion", type=int, default=2)
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--low-bits", type=int, default=0)
+    ap.add_argument("--reg-bits", type=int, default=4, help="register bits of the sweep layouts (3 or 4)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -414,6 +457,8 @@ def main() -> int:
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--precision-qubits", type=int, default=32)
     ap.add_argument("--precision-depth", type=int, default=60)
+    ap.add_argument("--precision-modes", default="",
+                    help="comma list of complex64 fusion modes to time as well: f2,f3,f3all,f3nopair,f3nopair_all")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -432,6 +477,7 @@ def main() -> int:
     log(f"[bench] workload random_blocks n={n} depth={args.depth} gates={gates}")
 
     eng = _core.Engine(n, args.fusion, args.tile_bits, args.low_bits)
+    eng.set_reg_bits(args.reg_bits)
     eng.load(text)
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -462,7 +508,7 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(n),
                 "traffic_source": f"profiles/tile_pass_{n}q_traffic.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE of this "
                                   "kernel from a separate counter run (not measured in this run)",
-                "kernel": "k_tile_reg<double,12>", "avg_launch_ms": round(avg_launch_ms, 4),
+                "kernel": "k_tile_reg8<double,12>" if args.reg_bits == 3 else "k_tile_reg<double,12>", "avg_launch_ms": round(avg_launch_ms, 4),
                 "launches_timed": int(timed), "sweeps_per_step": sweeps_per_step,
                 "kernel_time_frac_of_step": round(kernel_ms / 1e3 / elapsed, 4)}
 
@@ -476,7 +522,8 @@ def main() -> int:
     prec = None
     if not args.no_precision:
         log("[bench] precision (complex64 + MFMA blocks vs complex128) ...")
-        prec = precision_leg(args.precision_qubits, args.precision_depth, args.seed, 1)
+        modes = [m for m in args.precision_modes.split(",") if m]
+        prec = precision_leg(args.precision_qubits, args.precision_depth, args.seed, 1, modes)
         log(f"[bench] precision {prec}")
 
     batch = None
@@ -516,7 +563,8 @@ def main() -> int:
         "config": {"workload": f"random_blocks {n}q complex128 depth {args.depth} "
                                "(61 gates/layer; measure+reset+conditional every 10 layers)",
                    "qubits": n, "depth": args.depth, "gates": gates, "seed": args.seed,
-                   "fusion": args.fusion, "measurement_record": records[0]},
+                   "fusion": args.fusion, "reg_bits": args.reg_bits,
+                   "measurement_record": records[0]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adj,

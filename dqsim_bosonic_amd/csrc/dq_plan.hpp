@@ -640,6 +640,7 @@ struct RegPlan {
     std::vector<int> layouts;  // distinct layouts of the pass (entries of its layout table)
     int fused_blocks = 0;  // K_FUSED ops formed in this pass
     int fused_gates = 0;   // gates they replace
+    int relayouts = 0;     // LDS layout changes before ops, the store's included
 };
 
 // Register-layout order of a pass's ops (fusion level 2 and up).  A layout
@@ -871,6 +872,7 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
     };
     pd.load_lay = lay_index(pd.load_rb);
     pd.store_lay = lay_index(pd.store_rb);
+    out.relayouts = int(std::count(relay.begin(), relay.end(), 1)) + (last != pd.store_rb ? 1 : 0);
 
     for (size_t i = 0; i < nops; ++i) {
         const HostOp &h = *eff[i];

@@ -407,6 +407,35 @@ PYBIND11_MODULE(_core, m) {
         return out;
     }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
        py::arg("reg_tiles") = true);
+    // The register-tile plan of an op list (schedule, then plan_pass per
+    // pass): totals of passes, tile ops and LDS layout changes (the stores'
+    // included).
+    m.def("_plan_stats", [](int n, py::list ops, int fuse, int tile_bits, int low_bits, int reg_bits) {
+        std::vector<dq::HostOp> queue = host_ops(ops);
+        dq::PlanConfig cfg;
+        cfg.n = n;
+        cfg.fuse = fuse;
+        cfg.tile_bits = tile_bits;
+        cfg.low_bits = low_bits;
+        cfg.reg_bits = reg_bits;
+        const bool anywhere = fuse && dq::effective_tile_bits(&cfg) >= 9;
+        py::dict d;
+        long passes = 0, tops = 0, relay = 0;
+        std::vector<double> mats;
+        for (const dq::Pass &p : dq::schedule(&cfg, queue, anywhere)) {
+            ++passes;
+            if (!p.S || !anywhere) continue;
+            dq::RegPlan rp;
+            dq::plan_pass(&cfg, queue, p, mats, rp);
+            tops += long(rp.ops.size());
+            relay += rp.relayouts;
+        }
+        d["passes"] = passes;
+        d["tile_ops"] = tops;
+        d["relayouts"] = relay;
+        return d;
+    }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
+       py::arg("reg_bits") = 4);
     // The register-layout order of one pass's ops (dq_plan.hpp:
     // order_for_layouts); qubits not in tile_qubits are outer.  Returns the
     // op indices in the order the pass would apply them.

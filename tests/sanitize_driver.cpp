@@ -8,11 +8,14 @@
 // qubits, low qubits, register bits) it decodes the circuit, builds each
 // unitary's matrix, classifies it, splits the op stream at measurements,
 // schedules the passes, plans every pass (pair fusion, 5-qubit blocks,
-// layout order, layouts) and encodes the layout tables, then checks the
-// plan's invariants: every op lands in exactly one pass, register slots are
-// below the layout's register bits and distinct, a tile holds the configured
-// number of qubits.  Exit status 0 means no sanitizer report and no broken
+// layout order, layouts) and encodes the layout tables,
+// then checks the plan's invariants: every op lands in exactly one pass,
+// register slots are below the layout's register bits and distinct, a tile
+// holds the configured number of qubits, and an emulation of the tile's
+// amplitudes through the pass (emulate_pass) agrees with the kernel's view of
+// every layout.  Exit status 0 means no sanitizer report and no broken
 // invariant.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -58,6 +61,95 @@ bool unitary(const dqh::Instr &in, dqh::Mat &m) {
     return dqh::gate_matrix(in.kind, in.params, m);
 }
 
+// The kernel's layout formulas (dq_tile.hpp: make_layout and Layout::jp),
+// restated for the emulation below: register j of thread t holds tile index
+// tpart(t) | jp(j); the thread bits go, ascending, to the non-register tile
+// bits.
+unsigned tpart_of(int word, unsigned t, int R) {
+    unsigned tp = t;
+    for (int m = 0; m < R; ++m) {
+        const unsigned lo = (1u << ((word >> (4 * m)) & 15)) - 1;
+        tp = (tp & lo) | ((tp & ~lo) << 1);
+    }
+    return tp;
+}
+
+unsigned jp_of(int word, int j, int R) {
+    unsigned v = 0;
+    for (int m = 0; m < R; ++m)
+        if ((j >> m) & 1) v |= 1u << ((word >> (4 * m)) & 15);
+    return v;
+}
+
+// Emulates where every amplitude of one tile sits (thread, register) through
+// a planned pass -- the load, the LDS relayouts (each amplitude written at
+// and read back from its tile index), the store -- as the kernel runs it
+// (dq_tile.hpp: tile_sweep), and checks: the register bits of a layout are
+// packed ascending (make_layout inserts them in that order) and distinct;
+// every layout puts each tile index in exactly one (thread, register); the
+// register ops find their tile bits in the slots they name; the load and
+// store layouts keep the low tile bits on thread bits 0..low-1 (coalesced
+// runs).
+void emulate_pass(const dq::RegPlan &rp, unsigned lowmask) {
+    const int R = rp.reg_bits, b = rp.b;
+    const unsigned nt = 1u << (b - R), na = 1u << R;
+    std::vector<unsigned> held(size_t(nt) * na);
+    auto at = [&](unsigned t, unsigned j) -> unsigned & { return held[size_t(t) * na + j]; };
+    auto ascending = [&](int word) {
+        for (int m = 1; m < R; ++m)
+            if (((word >> (4 * m)) & 15) <= ((word >> (4 * (m - 1))) & 15)) return false;
+        return ((word >> (4 * (R - 1))) & 15) < b;
+    };
+    auto set_layout = [&](int word) {
+        for (unsigned t = 0; t < nt; ++t)
+            for (unsigned j = 0; j < na; ++j) at(t, j) = tpart_of(word, t, R) | jp_of(word, int(j), R);
+    };
+    auto bijective = [&]() {
+        std::vector<char> seen(size_t(1) << b, 0);
+        for (unsigned i : held) {
+            if (i >= (1u << b) || seen[i]) return false;
+            seen[i] = 1;
+        }
+        return true;
+    };
+    auto coalesced = [&](int word) {
+        for (unsigned t = 0; t < nt; ++t)
+            if ((tpart_of(word, t, R) & lowmask) != (t & lowmask)) return false;
+        for (int j = 0; j < int(na); ++j)
+            if (jp_of(word, j, R) & lowmask) return false;
+        return true;
+    };
+    auto slot_holds = [&](int slot, int bit) {
+        if (slot < 0 || slot >= R || bit < 0 || bit >= b) return false;
+        for (unsigned t = 0; t < nt; ++t)
+            for (unsigned j = 0; j < na; ++j)
+                if (((at(t, j) >> bit) & 1) != ((j >> slot) & 1)) return false;
+        return true;
+    };
+    EXPECT(ascending(rp.pd.load_rb) && coalesced(rp.pd.load_rb));
+    set_layout(rp.pd.load_rb);
+    EXPECT(bijective());
+    int cur = rp.pd.load_rb;
+    for (const dq::TileOp &op : rp.ops) {
+        if (op.relayout) {
+            EXPECT(ascending(op.rb));
+            set_layout(op.rb);
+            EXPECT(bijective());
+            cur = op.rb;
+        }
+        EXPECT(op.rb == cur);
+        if (op.kind == dq::T_DENSE1) EXPECT(slot_holds(op.p, op.bit[0]));
+        if (op.kind == dq::T_DENSE2) EXPECT(slot_holds(op.p, op.bit[0]) && slot_holds(op.q, op.bit[1]));
+        if (op.kind == dq::T_CTRL1) EXPECT(slot_holds(op.p, op.bit[op.k - 1]));
+        if (op.kind == dq::T_DIAGT && op.p >= 0) EXPECT(slot_holds(op.p, op.bit[op.q]));
+    }
+    // the kernel moves to the store layout through LDS when the last layout differs
+    EXPECT(ascending(rp.pd.store_rb) && coalesced(rp.pd.store_rb));
+    EXPECT(rp.relayouts == int(std::count_if(rp.ops.begin(), rp.ops.end(),
+                                             [](const dq::TileOp &o) { return o.relayout != 0; })) +
+                               (cur != rp.pd.store_rb ? 1 : 0));
+}
+
 void check_plan(const dq::PlanConfig &cfg, const std::vector<dq::HostOp> &queue) {
     const bool anywhere = cfg.fuse && cfg.reg_tiles && dq::effective_tile_bits(&cfg) >= 9;
     const std::vector<dq::Pass> passes = dq::schedule(&cfg, queue, anywhere);
@@ -79,6 +171,8 @@ void check_plan(const dq::PlanConfig &cfg, const std::vector<dq::HostOp> &queue)
             EXPECT(t.mat >= 0 && size_t(t.mat) * 2 <= mats.size());
             EXPECT(t.lay >= 0 && size_t(t.lay) < rp.layouts.size());
         }
+        const int low = std::max(0, std::min(cfg.low_bits, rp.b - R));
+        emulate_pass(rp, (1u << low) - 1);
         std::vector<uint32_t> table;
         for (int rb : rp.layouts) dq::append_layout_table(table, rb, rp.b, 16u, R);
         EXPECT(table.size() == rp.layouts.size() * (size_t(1) << R));

@@ -149,3 +149,4 @@ def test_eight_amplitude_layouts_equal_sixteen(n):
         ref = OracleSimulator(seed=17).simulate(circ)
         assert {int(k): v for k, v in r3.items()} == ref.classical_bits
         assert np.max(np.abs(e3.result().statevector - ref.statevector)) <= 1e-12
+
