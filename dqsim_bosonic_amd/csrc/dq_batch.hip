@@ -253,6 +253,7 @@ __global__ __launch_bounds__(kPersistThreads, 2) void k_batch_persist(
     double *__restrict__ out) {
     __shared__ int s_item;
     __shared__ double s_red[kPersistThreads / 64];
+    extern __shared__ __align__(16) unsigned char tile_lds[];
     cplx<T> *amps = arena + uint64_t(blockIdx.x) * slot_amps;
     const unsigned t = threadIdx.x;
     for (;;) {
@@ -269,7 +270,8 @@ __global__ __launch_bounds__(kPersistThreads, 2) void k_batch_persist(
         __syncthreads();
         for (int p = 0; p < item.pass_count; ++p) {
             const TilePass &pd = passes[item.pass_begin + p];
-            tile_sweep<T, kPersistTileBits, 1, 4>(amps, pd, ops, mats, uint64_t(1) << pd.n_outer, nullptr, 0, 1);
+            tile_sweep<T, kPersistTileBits, 1, 4>(amps, pd, ops, mats, uint64_t(1) << pd.n_outer, nullptr, 0, 1,
+                                                  tile_lds);
             __threadfence();
             __syncthreads();
         }

@@ -459,14 +459,15 @@ __device__ __forceinline__ void store_nt(cplx<T> *p, cplx<T> v) {
 //
 // The sweep walks tiles vblock, vblock + vgrid, ... (vgrid a power of two):
 // a launch passes blockIdx.x / gridDim.x; a workgroup that owns a whole state
-// passes 0 / 1 and walks every tile in order.
+// passes 0 / 1 and walks every tile in order.  smem_raw: the tile's LDS, 2^B
+// amplitudes (the sweep kernels declare it statically, so its address is a
+// constant folded into each LDS access).
 template <typename T, int B, int FAST, int R>
 __device__ __forceinline__ void tile_sweep(cplx<T> *__restrict__ amps, const TilePass &pd,
                                            const TileOp *__restrict__ ops, const cplx<T> *__restrict__ mats,
                                            uint64_t ntiles, double *__restrict__ partials, uint64_t vblock,
-                                           uint64_t vgrid) {
+                                           uint64_t vgrid, unsigned char *smem_raw) {
     double red = 0.0;  // this thread's share of the epilogue reduction
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
     // layout tables of this pass: 16 LDS byte offsets per layout
     const unsigned *laytab =
@@ -741,7 +742,8 @@ __global__ __launch_bounds__(1 << (B - 4)) void k_tile_reg(cplx<T> *__restrict__
                                                            const TileOp *__restrict__ ops,
                                                            const cplx<T> *__restrict__ mats, uint64_t ntiles,
                                                            double *__restrict__ partials) {
-    tile_sweep<T, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x);
+    __shared__ __align__(16) unsigned char tile_lds[sizeof(cplx<T>) << B];
+    tile_sweep<T, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x, tile_lds);
 }
 
 // Eight amplitudes per thread (3 register bits): twice the threads of
@@ -754,7 +756,8 @@ __global__ __launch_bounds__(1 << (B - 3), 4) void k_tile_reg8(cplx<T> *__restri
                                                                const TileOp *__restrict__ ops,
                                                                const cplx<T> *__restrict__ mats, uint64_t ntiles,
                                                                double *__restrict__ partials) {
-    tile_sweep<T, B, FAST, 3>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x);
+    __shared__ __align__(16) unsigned char tile_lds[sizeof(cplx<T>) << B];
+    tile_sweep<T, B, FAST, 3>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x, tile_lds);
 }
 
 // complex64 without the MFMA block op (FAST = 1): the sweep fits 128 VGPRs,
@@ -766,7 +769,8 @@ __global__ __launch_bounds__(1 << (B - 4), 4) void k_tile_reg_c64(cplx<float> *_
                                                                   const TileOp *__restrict__ ops,
                                                                   const cplx<float> *__restrict__ mats,
                                                                   uint64_t ntiles, double *__restrict__ partials) {
-    tile_sweep<float, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x);
+    __shared__ __align__(16) unsigned char tile_lds[sizeof(cplx<float>) << B];
+    tile_sweep<float, B, FAST, 4>(amps, pd, ops, mats, ntiles, partials, blockIdx.x, gridDim.x, tile_lds);
 }
 
 }  // namespace dq

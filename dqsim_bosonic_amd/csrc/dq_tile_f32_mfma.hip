@@ -11,13 +11,7 @@ template <int B>
 hipError_t launch(unsigned grid, hipStream_t stream, void *amps, const TilePass &pd, const TileOp *ops,
                   const void *mats, uint64_t ntiles, double *partials) {
     using T = float;
-    const size_t shmem = sizeof(cplx<T>) << B;
-    if (shmem > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_reg<T, B, 2>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem));
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((k_tile_reg<T, B, 2>), dim3(grid), dim3(1 << (B - 4)), shmem, stream,
+    hipLaunchKernelGGL((k_tile_reg<T, B, 2>), dim3(grid), dim3(1 << (B - 4)), 0, stream,
                        static_cast<cplx<T> *>(amps), pd, ops, static_cast<const cplx<T> *>(mats), ntiles, partials);
     return hipGetLastError();
 }
