@@ -436,6 +436,41 @@ PYBIND11_MODULE(_core, m) {
         return d;
     }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
        py::arg("reg_bits") = 4);
+    // The register-tile plan of an op list, pass by pass: tile bits, load and
+    // store layouts, and each tile op as (kind, variant, relayout, layout,
+    // register slots p and q, tile bits).
+    m.def("_plan_passes", [](int n, py::list ops, int fuse, int tile_bits, int low_bits, int reg_bits) {
+        std::vector<dq::HostOp> queue = host_ops(ops);
+        dq::PlanConfig cfg;
+        cfg.n = n;
+        cfg.fuse = fuse;
+        cfg.tile_bits = tile_bits;
+        cfg.low_bits = low_bits;
+        cfg.reg_bits = reg_bits;
+        const bool anywhere = fuse && dq::effective_tile_bits(&cfg) >= 9;
+        py::list out;
+        std::vector<double> mats;
+        for (const dq::Pass &p : dq::schedule(&cfg, queue, anywhere)) {
+            if (!p.S || !anywhere) continue;
+            dq::RegPlan rp;
+            dq::plan_pass(&cfg, queue, p, mats, rp);
+            py::list tops;
+            for (const dq::TileOp &t : rp.ops) {
+                py::list bits;
+                for (int j = 0; j < t.k; ++j) bits.append(t.bit[j]);
+                tops.append(py::make_tuple(t.kind, t.variant, t.relayout, t.rb, t.p, t.q, bits));
+            }
+            py::dict d;
+            d["b"] = rp.b;
+            d["load_rb"] = rp.pd.load_rb;
+            d["store_rb"] = rp.pd.store_rb;
+            d["relayouts"] = rp.relayouts;
+            d["ops"] = tops;
+            out.append(d);
+        }
+        return out;
+    }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
+       py::arg("reg_bits") = 4);
     // The register-layout order of one pass's ops (dq_plan.hpp:
     // order_for_layouts); qubits not in tile_qubits are outer.  Returns the
     // op indices in the order the pass would apply them.
