@@ -10,7 +10,8 @@ remote_link_psi_plus + 8 nonlocal_cz across the two 15-qubit halves), with
 One step = reset the resident device state to |0...0> and run the whole
 circuit through the dispatcher (gates, mid-circuit measurements with host
 ChaCha8 draws, resets, conditional); the circuit is decoded once, before
-timing.  value = circuit gates per second over the K timed steps.
+timing.  value = gates applied per second over the K timed steps (a
+conditional gate counts when its condition held).
 
 roofline: the dominant kernel is the LDS-tiled sweep (k_tile_reg); its
 launches are bracketed by HIP events on the engine's stream.  Algorithmic
@@ -499,7 +500,18 @@ def main() -> int:
 
     elapsed = t1 - t0
     ms_per_step = 1000.0 * elapsed / args.steps
-    value = gates * args.steps / elapsed
+    # gates applied per step: a conditional's gate counts only when its
+    # condition held (the record is the same in every step)
+    skipped = 0
+    for ins in circ.instructions:
+        if ins.kind != "conditional":
+            continue
+        c = ins.model_dump()["condition"]
+        val = sum(int(records[0].get(c["creg_base"] + b, 0)) << b for b in range(c["creg_size"]))
+        if val != c["creg_value"]:
+            skipped += 1
+    applied = gates - skipped
+    value = applied * args.steps / elapsed
     sweeps_per_step = (sweeps1 - sweeps0) / args.steps
     avg_launch_ms = kernel_ms / max(timed, 1)
     alg_bytes = 2.0 * 16.0 * float(1 << n)
@@ -562,7 +574,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"random_blocks {n}q complex128 depth {args.depth} "
                                "(61 gates/layer; measure+reset+conditional every 10 layers)",
-                   "qubits": n, "depth": args.depth, "gates": gates, "seed": args.seed,
+                   "qubits": n, "depth": args.depth, "gates": gates, "gates_applied_per_step": applied,
+                   "seed": args.seed,
                    "fusion": args.fusion, "reg_bits": args.reg_bits,
                    "measurement_record": records[0]},
         "roofline": roofline,

@@ -355,6 +355,22 @@ __device__ __forceinline__ double wave_sum_lane63(double v) {
 template <typename T>
 using amp2 = T __attribute__((ext_vector_type(2)));
 
+// Complex products as FMA chains: u * x is two products and two FMAs, and
+// acc + u * x four FMAs (the kernel is compiled with contraction).
+template <typename T>
+__device__ __forceinline__ T afma(T a, T b, T c) {
+    if constexpr (sizeof(T) == 8) return __builtin_fma(a, b, c);
+    else return __builtin_fmaf(a, b, c);
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cfmul(cplx<T> u, cplx<T> x) {
+    return cplx<T>{afma(u.re, x.re, -(u.im * x.im)), afma(u.re, x.im, u.im * x.re)};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cfmac(cplx<T> acc, cplx<T> u, cplx<T> x) {
+    return cplx<T>{afma(-u.im, x.im, afma(u.re, x.re, acc.re)), afma(u.im, x.re, afma(u.re, x.im, acc.im))};
+}
+
 template <typename T, int NR, int M>
 __device__ __forceinline__ void r_dense1(cplx<T> (&a)[NR], const cplx<T> u0, const cplx<T> u1, const cplx<T> u2,
                                          const cplx<T> u3) {
@@ -363,8 +379,8 @@ __device__ __forceinline__ void r_dense1(cplx<T> (&a)[NR], const cplx<T> u0, con
         if (j & (1 << M)) continue;
         const int j1 = j | (1 << M);
         const cplx<T> x0 = a[j], x1 = a[j1];
-        a[j] = cadd(cmul(u0, x0), cmul(u1, x1));
-        a[j1] = cadd(cmul(u2, x0), cmul(u3, x1));
+        a[j] = cfmac(cfmul(u1, x1), u0, x0);
+        a[j1] = cfmac(cfmul(u3, x1), u2, x0);
     }
 }
 
@@ -394,14 +410,14 @@ __device__ __forceinline__ void r_pair_inner(const cplx<T> (&p)[NR], const cplx<
         const int j1 = j | (1 << M);
         const double l0r = double(l[j].re), l0i = double(l[j].im), l1r = double(l[j1].re), l1i = double(l[j1].im);
         const double p0r = double(p[j].re), p0i = double(p[j].im), p1r = double(p[j1].re), p1i = double(p[j1].im);
-        ar += l1r * p0r + l1i * p0i;
-        ai += l1r * p0i - l1i * p0r;
+        ar = afma(l1i, p0i, afma(l1r, p0r, ar));
+        ai = afma(-l1i, p0r, afma(l1r, p0i, ai));
         if (Y) {
-            ar -= l0r * p1r + l0i * p1i;
-            ai -= l0r * p1i - l0i * p1r;
+            ar = afma(-l0i, p1i, afma(-l0r, p1r, ar));
+            ai = afma(l0i, p1r, afma(-l0r, p1i, ai));
         } else {
-            ar += l0r * p1r + l0i * p1i;
-            ai += l0r * p1i - l0i * p1r;
+            ar = afma(l0i, p1i, afma(l0r, p1r, ar));
+            ai = afma(-l0i, p1r, afma(l0r, p1i, ai));
         }
     }
 }
@@ -417,9 +433,9 @@ __device__ __forceinline__ void r_dense2(cplx<T> (&a)[NR], const cplx<T> (&u)[16
         for (int e = 0; e < 4; ++e) x[e] = a[idx[e]];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            cplx<T> acc = cmul(u[4 * r], x[0]);
+            cplx<T> acc = cfmul(u[4 * r + 3], x[3]);
 #pragma unroll
-            for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(u[4 * r + c], x[c]));
+            for (int c = 2; c >= 0; --c) acc = cfmac(acc, u[4 * r + c], x[c]);
             a[idx[r]] = acc;
         }
     }
