@@ -305,7 +305,9 @@ enum AdjRKind : int {
     R_RELAYOUT = 4,
     R_DTABLE = 5,
     R_DENSE1R = 6,  // 2x2 with real entries on register slot a
-    R_GRAD1 = 7     // one term, X (b = 0) or Y (b = 1) on register slot a alone
+    R_GRAD1 = 7,    // one term, X (b = 0) or Y (b = 1) on register slot a alone
+    R_ZGRADS = 8    // nterms (<= 8) gradients of one Z-string term each, terms term0.., the slot of
+                    // gradient g in its term's `slot`
 };
 
 struct AdjRStep {
@@ -322,7 +324,7 @@ struct AdjRStep {
 struct AdjRTerm {
     unsigned xr;       // X bits as register-slot mask
     unsigned xt, z;    // X bits and Z bits as tile-bit masks
-    unsigned pad;
+    int slot;          // R_ZGRADS: the gradient's partial-sum slot
     double cre, cim;   // coefficient times i^{#Y}
 };
 
@@ -349,6 +351,63 @@ __device__ __forceinline__ double wave_sum_lane63(double v) {
     v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
     v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
     return v;
+}
+
+// One 32-bit-word exchange step of a double pair across lane bit C (C = 5, 4:
+// v_permlane32_swap / v_permlane16_swap; C = 3: DPP row shifts by 8 with bank
+// masks): afterwards lanes whose bit C is 0 hold x of both lanes of the pair
+// and lanes whose bit C is 1 hold y of both, so x + y sums x over the pair in
+// the first and y in the second.
+template <int C>
+__device__ __forceinline__ double fold_pair(double x, double y) {
+    const long long xb = __double_as_longlong(x), yb = __double_as_longlong(y);
+    unsigned xl = unsigned(xb), xh = unsigned(xb >> 32), yl = unsigned(yb), yh = unsigned(yb >> 32);
+    if constexpr (C == 5) {
+        auto r = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+        xl = r[0];
+        yl = r[1];
+        r = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+        xh = r[0];
+        yh = r[1];
+    } else if constexpr (C == 4) {
+        auto r = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+        xl = r[0];
+        yl = r[1];
+        r = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+        xh = r[0];
+        yh = r[1];
+    } else {
+        const unsigned nxl = __builtin_amdgcn_update_dpp(xl, yl, 0x118, 0xF, 0xC, false);  // row_shr:8, banks 2-3
+        const unsigned nyl = __builtin_amdgcn_update_dpp(yl, xl, 0x108, 0xF, 0x3, false);  // row_shl:8, banks 0-1
+        const unsigned nxh = __builtin_amdgcn_update_dpp(xh, yh, 0x118, 0xF, 0xC, false);
+        const unsigned nyh = __builtin_amdgcn_update_dpp(yh, xh, 0x108, 0xF, 0x3, false);
+        xl = nxl;
+        yl = nyl;
+        xh = nxh;
+        yh = nyh;
+    }
+    const double nx = __longlong_as_double((static_cast<long long>(xh) << 32) | xl);
+    const double ny = __longlong_as_double((static_cast<long long>(yh) << 32) | yl);
+    return nx + ny;
+}
+
+// Sums of v[0..7] over the 64 lanes of a wave in one butterfly: three folds
+// (lane bits 5, 4, 3) leave in each lane the pair sums of value
+// 4 bit3 + 2 bit4 + bit5 of its lane index, then two quad permutes and a row
+// shift by 4 add the 8 lanes of each group (bits 0..2).  The total of value i
+// ends in lane 4 + 8 (4 (i & 1) + 2 ((i >> 1) & 1) + (i >> 2)) (lanes with
+// bits 0..2 = 4); returns it there.
+__device__ __forceinline__ double wave_sum8(const double (&v)[8]) {
+    const double w0 = fold_pair<5>(v[0], v[1]), w1 = fold_pair<5>(v[2], v[3]);
+    const double w2 = fold_pair<5>(v[4], v[5]), w3 = fold_pair<5>(v[6], v[7]);
+    const double u0 = fold_pair<4>(w0, w1), u1 = fold_pair<4>(w2, w3);
+    double t = fold_pair<3>(u0, u1);
+    t += dpp_f64<0xB1, 0xf>(t);  // quad_perm [1,0,3,2]
+    t += dpp_f64<0x4E, 0xf>(t);  // quad_perm [2,3,0,1]
+    const long long tb = __double_as_longlong(t);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(tb & 0xffffffff), 0x114, 0xf, 0xa, false);  // row_shr:4
+    const int hi = __builtin_amdgcn_update_dpp(0, int(tb >> 32), 0x114, 0xf, 0xa, false);
+    return t + __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
 
 // An amplitude as one naturally aligned vector: one LDS access each.
@@ -705,6 +764,43 @@ __global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoi
                 }
                 break;
             }
+            case R_ZGRADS: {
+                // conj(l) p once for the group: every gradient of a Z string
+                // is s (cre Im(conj(l) p) + cim Re(conj(l) p)) summed, with
+                // the sign s split into a per-lane and a per-register factor
+                double wr[NR], wi[NR];
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    const double pr = double(p[j].re), pi = double(p[j].im);
+                    const double lr = double(l[j].re), li = double(l[j].im);
+                    wr[j] = afma(li, pi, lr * pr);
+                    wi[j] = afma(-li, pr, lr * pi);
+                }
+                double v[8];
+#pragma unroll
+                for (int g = 0; g < 8; ++g) {
+                    v[g] = 0.0;
+                    if (g < st.nterms) {
+                        const AdjRTerm tm = terms[st.term0 + g];
+                        double im_sum = 0.0, re_sum = 0.0;
+#pragma unroll
+                        for (int j = 0; j < NR; ++j) {
+                            const double sj = (__popc(tm.z & reg_bits(j)) & 1) ? -1.0 : 1.0;  // scalar
+                            im_sum = afma(sj, wi[j], im_sum);
+                            re_sum = afma(sj, wr[j], re_sum);
+                        }
+                        const double lane_sign = (__popc(tm.z & tpart) & 1) ? -1.0 : 1.0;
+                        v[g] = lane_sign * afma(tm.cre, im_sum, tm.cim * re_sum);
+                    }
+                }
+                const double tot = wave_sum8(v);
+                if ((lane & 7) == 4) {
+                    const int g = 4 * int((lane >> 3) & 1) + 2 * int((lane >> 4) & 1) + int((lane >> 5) & 1);
+                    if (g < st.nterms)
+                        partials[(uint64_t(terms[st.term0 + g].slot) * ntiles + tile) * W + wave] = tot;
+                }
+                break;
+            }
             default: {  // R_GRAD
                 double acc = 0.0;
                 if (st.a) {
@@ -1028,7 +1124,36 @@ bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm>
         }
         i = j;
     }
-    out.swap(res);
+    // Consecutive one-term Z-string gradients, up to 8 at a time, become one
+    // R_ZGRADS step: conj(lam) psi is formed once and the 8 wave sums are one
+    // butterfly.  Their terms are copied to consecutive entries.
+    std::vector<AdjRStep> grouped(res.begin(), res.begin() + long(out0));
+    for (size_t i2 = out0; i2 < res.size();) {
+        const auto single_z = [&](const AdjRStep &x) { return x.kind == R_GRAD && x.a == 1 && x.nterms == 1; };
+        if (!single_z(res[i2])) {
+            grouped.push_back(res[i2++]);
+            continue;
+        }
+        size_t j = i2;
+        while (j < res.size() && j - i2 < 8 && single_z(res[j])) ++j;
+        if (j - i2 == 1) {
+            grouped.push_back(res[i2++]);
+            continue;
+        }
+        AdjRStep gz;
+        std::memset(&gz, 0, sizeof gz);
+        gz.kind = R_ZGRADS;
+        gz.term0 = int(rterms.size());
+        gz.nterms = int(j - i2);
+        for (size_t q = i2; q < j; ++q) {
+            AdjRTerm t = rterms[size_t(res[q].term0)];
+            t.slot = res[q].slot;
+            rterms.push_back(t);
+        }
+        grouped.push_back(gz);
+        i2 = j;
+    }
+    out.swap(grouped);
     return true;
 }
 
