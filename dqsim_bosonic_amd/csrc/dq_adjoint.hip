@@ -306,8 +306,13 @@ enum AdjRKind : int {
     R_DTABLE = 5,
     R_DENSE1R = 6,  // 2x2 with real entries on register slot a
     R_GRAD1 = 7,    // one term, X (b = 0) or Y (b = 1) on register slot a alone
-    R_ZGRADS = 8    // nterms (<= 8) gradients of one Z-string term each, terms term0.., the slot of
-                    // gradient g in its term's `slot`
+    R_ZGRADS = 8,   // nterms (<= 8) gradients of one Z-string term each, terms term0.., the slot of
+                    // gradient g in its term's `slot`; b = 1: then the phase table (mask, mat) of R_DTABLE
+    R_GRAD1R = 9,   // R_GRAD1 on slot a (b = 1 for Y, term term0, partial slot `slot`) followed by
+                    // R_DENSE1R on the same slot (entries at mat): a rotation's gradient and its inverse
+    R_ROT3 = 10     // three R_GRAD1R, on register slots 0, 1, 2 in that order, entry term0 + m for slot
+                    // m: 1 for Y in `z`, the real 2x2 at mats offset `xt`, partial slot `slot` (-1: no
+                    // gradient; the identity and a zero coefficient stand in for an absent rotation)
 };
 
 struct AdjRStep {
@@ -318,7 +323,8 @@ struct AdjRStep {
     int bits[2];       // DIAG: tile bits, MSB first
     int mat;           // matrix offset (complex entries)
     int term0, nterms, slot;  // GRAD
-    unsigned mask;     // RELAYOUT: register tile bits of the new layout; DTABLE: the table's tile bits
+    unsigned mask;     // RELAYOUT: register tile bits of the new layout; DTABLE (and ZGRADS with b = 1):
+                       // the table's tile bits
 };
 
 struct AdjRTerm {
@@ -481,6 +487,33 @@ __device__ __forceinline__ void r_pair_inner(const cplx<T> (&p)[NR], const cplx<
     }
 }
 
+// A rotation on register slot M inside a three-rotation step: the gradient
+// Im(c <lam| P |psi>) for P = X or Y (runtime, `z` of the entry) into g, then
+// the real 2x2 inverse on both states.  The sum over this thread's pairs is
+// split as A1 + s A0, s = -1 for Y, so that the Pauli choice costs no branch.
+template <typename T, int NR, int M>
+__device__ __forceinline__ void rot_slot(cplx<T> (&p)[NR], cplx<T> (&l)[NR], const AdjRTerm tm,
+                                         const cplx<double> *__restrict__ mats, double &g) {
+    double a1r = 0.0, a1i = 0.0, a0r = 0.0, a0i = 0.0;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        if (j & (1 << M)) continue;
+        const int j1 = j | (1 << M);
+        const double l0r = double(l[j].re), l0i = double(l[j].im), l1r = double(l[j1].re), l1i = double(l[j1].im);
+        const double p0r = double(p[j].re), p0i = double(p[j].im), p1r = double(p[j1].re), p1i = double(p[j1].im);
+        a1r = afma(l1i, p0i, afma(l1r, p0r, a1r));
+        a1i = afma(-l1i, p0r, afma(l1r, p0i, a1i));
+        a0r = afma(l0i, p1i, afma(l0r, p1r, a0r));
+        a0i = afma(-l0i, p1r, afma(l0r, p1i, a0i));
+    }
+    const double sg = tm.z ? -1.0 : 1.0;
+    const double ar = afma(sg, a0r, a1r), ai = afma(sg, a0i, a1i);
+    g = tm.cre * ai + tm.cim * ar;  // Im(c (ar + i ai))
+    const T u0 = T(mats[tm.xt].re), u1 = T(mats[tm.xt + 1].re), u2 = T(mats[tm.xt + 2].re), u3 = T(mats[tm.xt + 3].re);
+    r_dense1r<T, NR, M>(p, u0, u1, u2, u3);
+    r_dense1r<T, NR, M>(l, u0, u1, u2, u3);
+}
+
 template <typename T, int NR, int MH, int ML>
 __device__ __forceinline__ void r_dense2(cplx<T> (&a)[NR], const cplx<T> (&u)[16]) {
 #pragma unroll
@@ -631,6 +664,39 @@ __global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoi
             p[j] = psi[g];
             l[j] = lam[g];
         }
+        // a product of diagonal gates as one phase table over the tile bits in m
+        auto apply_dtable = [&](unsigned m, int mat) __attribute__((always_inline)) {
+            unsigned pt = 0;
+            {
+                unsigned mm = m, pos = 0;
+                while (mm) {
+                    const unsigned low = mm & (~mm + 1u);
+                    if (tpart & low) pt |= 1u << pos;
+                    ++pos;
+                    mm &= mm - 1u;
+                }
+            }
+            unsigned c[RB];
+            {
+                unsigned cm = cur_mask;
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    const unsigned low = cm & (~cm + 1u);
+                    c[r] = (m & low) ? (1u << __popc(m & (low - 1u))) : 0u;
+                    cm &= cm - 1u;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                unsigned row = pt;
+#pragma unroll
+                for (int r = 0; r < RB; ++r)
+                    if ((j >> r) & 1) row |= c[r];
+                const cplx<T> ph = cvt<T>(mats[mat + int(row)]);
+                p[j] = cmul(ph, p[j]);
+                l[j] = cmul(ph, l[j]);
+            }
+        };
         for (int s = 0; s < pd.nsteps; ++s) {
             const AdjRStep st = steps[pd.step0 + s];
             switch (st.kind) {
@@ -681,6 +747,24 @@ __global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoi
                 }
                 break;
             }
+            case R_GRAD1R: {
+                double ar = 0.0, ai = 0.0;
+                const T u0 = T(mats[st.mat].re), u1 = T(mats[st.mat + 1].re);
+                const T u2 = T(mats[st.mat + 2].re), u3 = T(mats[st.mat + 3].re);
+                switch (st.a * 2 + st.b) {
+                case 0: r_pair_inner<T, NR, 0, 0>(p, l, ar, ai); r_dense1r<T, NR, 0>(p, u0, u1, u2, u3); r_dense1r<T, NR, 0>(l, u0, u1, u2, u3); break;
+                case 1: r_pair_inner<T, NR, 0, 1>(p, l, ar, ai); r_dense1r<T, NR, 0>(p, u0, u1, u2, u3); r_dense1r<T, NR, 0>(l, u0, u1, u2, u3); break;
+                case 2: r_pair_inner<T, NR, 1 % RB, 0>(p, l, ar, ai); r_dense1r<T, NR, 1 % RB>(p, u0, u1, u2, u3); r_dense1r<T, NR, 1 % RB>(l, u0, u1, u2, u3); break;
+                case 3: r_pair_inner<T, NR, 1 % RB, 1>(p, l, ar, ai); r_dense1r<T, NR, 1 % RB>(p, u0, u1, u2, u3); r_dense1r<T, NR, 1 % RB>(l, u0, u1, u2, u3); break;
+                case 4: r_pair_inner<T, NR, 2 % RB, 0>(p, l, ar, ai); r_dense1r<T, NR, 2 % RB>(p, u0, u1, u2, u3); r_dense1r<T, NR, 2 % RB>(l, u0, u1, u2, u3); break;
+                default: r_pair_inner<T, NR, 2 % RB, 1>(p, l, ar, ai); r_dense1r<T, NR, 2 % RB>(p, u0, u1, u2, u3); r_dense1r<T, NR, 2 % RB>(l, u0, u1, u2, u3); break;
+                }
+                const AdjRTerm tm = terms[st.term0];
+                double acc = tm.cre * ai + tm.cim * ar;  // Im(c (ar + i ai))
+                acc = wave_sum_lane63(acc);
+                if (lane == 63) partials[(uint64_t(st.slot) * ntiles + tile) * W + wave] = acc;
+                break;
+            }
             case R_GRAD1: {
                 double ar = 0.0, ai = 0.0;
                 const int code = st.a * 2 + st.b;
@@ -713,41 +797,9 @@ __global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoi
                 }
                 break;
             }
-            case R_DTABLE: {
-                // a product of diagonal gates as one phase table over the tile bits in st.mask
-                const unsigned m = st.mask;
-                unsigned pt = 0;
-                {
-                    unsigned mm = m, pos = 0;
-                    while (mm) {
-                        const unsigned low = mm & (~mm + 1u);
-                        if (tpart & low) pt |= 1u << pos;
-                        ++pos;
-                        mm &= mm - 1u;
-                    }
-                }
-                unsigned c[RB];
-                {
-                    unsigned cm = cur_mask;
-#pragma unroll
-                    for (int r = 0; r < RB; ++r) {
-                        const unsigned low = cm & (~cm + 1u);
-                        c[r] = (m & low) ? (1u << __popc(m & (low - 1u))) : 0u;
-                        cm &= cm - 1u;
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < NR; ++j) {
-                    unsigned row = pt;
-#pragma unroll
-                    for (int r = 0; r < RB; ++r)
-                        if ((j >> r) & 1) row |= c[r];
-                    const cplx<T> ph = cvt<T>(mats[st.mat + int(row)]);
-                    p[j] = cmul(ph, p[j]);
-                    l[j] = cmul(ph, l[j]);
-                }
+            case R_DTABLE:
+                apply_dtable(st.mask, st.mat);
                 break;
-            }
             case R_DIAG: {
                 const int dim = 1 << st.k;
                 const cplx<T> d0 = cvt<T>(mats[st.mat]), d1 = cvt<T>(mats[st.mat + (dim + 1)]);
@@ -799,8 +851,25 @@ __global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoi
                     if (g < st.nterms)
                         partials[(uint64_t(terms[st.term0 + g].slot) * ntiles + tile) * W + wave] = tot;
                 }
+                if (st.b) apply_dtable(st.mask, st.mat);
                 break;
             }
+            case R_ROT3:
+                if constexpr (RB == 3) {
+                    // straight-line: the three slots in order, no branches, so
+                    // the amplitudes stay in place through the step
+                    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                    rot_slot<T, NR, 0>(p, l, terms[st.term0], mats, v[0]);
+                    rot_slot<T, NR, 1>(p, l, terms[st.term0 + 1], mats, v[1]);
+                    rot_slot<T, NR, 2>(p, l, terms[st.term0 + 2], mats, v[2]);
+                    const double tot = wave_sum8(v);
+                    const int i = 4 * int((lane >> 3) & 1) + 2 * int((lane >> 4) & 1) + int((lane >> 5) & 1);
+                    if ((lane & 7) == 4 && i < 3) {
+                        const int ps = terms[st.term0 + i].slot;
+                        if (ps >= 0) partials[(uint64_t(ps) * ntiles + tile) * W + wave] = tot;
+                    }
+                }
+                break;
             default: {  // R_GRAD
                 double acc = 0.0;
                 if (st.a) {
@@ -1153,7 +1222,80 @@ bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm>
         grouped.push_back(gz);
         i2 = j;
     }
-    out.swap(grouped);
+    // A rotation's pair-form gradient followed by its inverse on the same
+    // register slot (real entries) is one step.
+    std::vector<AdjRStep> merged(grouped.begin(), grouped.begin() + long(out0));
+    for (size_t i3 = out0; i3 < grouped.size(); ++i3) {
+        const AdjRStep &g = grouped[i3];
+        if (g.kind == R_GRAD1 && i3 + 1 < grouped.size() && grouped[i3 + 1].kind == R_DENSE1R &&
+            grouped[i3 + 1].a == g.a) {
+            AdjRStep m = g;
+            m.kind = R_GRAD1R;
+            m.mat = grouped[i3 + 1].mat;
+            merged.push_back(m);
+            ++i3;
+            continue;
+        }
+        merged.push_back(g);
+    }
+    // Consecutive rotation steps on distinct register slots (2 or 3 of them)
+    // become one R_ROT3 step: rotations on different qubits commute, so they
+    // run in slot order, and an absent slot takes the identity with no
+    // gradient.  A Z-gradient group absorbs the phase table that follows it.
+    int ident = -1;  // mats offset of the real 2x2 identity
+    std::vector<AdjRStep> fused(merged.begin(), merged.begin() + long(out0));
+    for (size_t i4 = out0; i4 < merged.size();) {
+        const AdjRStep &g = merged[i4];
+        if (g.kind == R_ZGRADS && i4 + 1 < merged.size() && merged[i4 + 1].kind == R_DTABLE) {
+            AdjRStep z = g;
+            z.b = 1;
+            z.mask = merged[i4 + 1].mask;
+            z.mat = merged[i4 + 1].mat;
+            fused.push_back(z);
+            i4 += 2;
+            continue;
+        }
+        size_t j = i4;
+        unsigned used = 0;
+        while (j < merged.size() && merged[j].kind == R_GRAD1R && !((used >> merged[j].a) & 1) && RB == 3) {
+            used |= 1u << merged[j].a;
+            ++j;
+        }
+        if (j - i4 < 2) {
+            fused.push_back(g);
+            ++i4;
+            continue;
+        }
+        if (ident < 0) {
+            ident = int(hmats.size());
+            hmats.push_back(cplx<double>{1.0, 0.0});
+            hmats.push_back(cplx<double>{0.0, 0.0});
+            hmats.push_back(cplx<double>{0.0, 0.0});
+            hmats.push_back(cplx<double>{1.0, 0.0});
+        }
+        AdjRStep r;
+        std::memset(&r, 0, sizeof r);
+        r.kind = R_ROT3;
+        r.term0 = int(rterms.size());
+        r.nterms = 3;
+        for (int m = 0; m < 3; ++m) {
+            AdjRTerm t;
+            std::memset(&t, 0, sizeof t);
+            t.xt = unsigned(ident);
+            t.slot = -1;
+            for (size_t q = i4; q < j; ++q)
+                if (merged[q].a == m) {
+                    t = rterms[size_t(merged[q].term0)];
+                    t.z = unsigned(merged[q].b);
+                    t.xt = unsigned(merged[q].mat);
+                    t.slot = merged[q].slot;
+                }
+            rterms.push_back(t);
+        }
+        fused.push_back(r);
+        i4 = j;
+    }
+    out.swap(fused);
     return true;
 }
 
