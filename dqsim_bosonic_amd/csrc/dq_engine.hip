@@ -1071,6 +1071,20 @@ int dq_collapse(dq_state *s, int qubit, int outcome, double norm) {
 int dq_prob_one(dq_state *s, int qubit, double *p1) {
     if (!s || !p1) return fail(DQ_ERR_ARG, "null argument");
     if (int rc = check_qubit(s, qubit)) return rc;
+    if (!s->queue.empty() && all_elementwise(s)) {
+        // a pending collapse of this qubit to 0, with only diagonal ops and
+        // collapses after it, leaves every amplitude with the bit set at an
+        // exact zero: the sum is +0.0 without reading the state (a reset
+        // right after a measurement that gave 0)
+        for (auto it = s->queue.rbegin(); it != s->queue.rend(); ++it) {
+            if (it->kind != K_COLLAPSE || it->q[0] != qubit) continue;
+            if (it->outcome == 0) {
+                *p1 = 0.0;
+                return DQ_OK;
+            }
+            break;
+        }
+    }
     const uint64_t count = amp_count(s) / 2;
     const int blocks = std::min<int>(kReduceBlocks, grid_for(count));
     int nparts = blocks;
