@@ -206,6 +206,43 @@ __device__ __forceinline__ void reg_ctrl1(cplx<T> (&a)[1 << R], const cplx<T> *_
 template <typename T, int P, int Q, int V, int FAST, int R>
 __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[1 << R], const cplx<T> *__restrict__ m) {
     constexpr int LO = P < Q ? P : Q, HI = P < Q ? Q : P;
+    if constexpr (FAST && V != V_DIAG) {
+        // Contracted: rows 0-1 of every group, then rows 2-3, so that only
+        // two matrix rows (32 scalar registers for complex128) are live at
+        // once; with all four rows live the scalar registers spill and every
+        // op reloads them (10-16 v_readlane per op).  Per output the FMA
+        // chain is unchanged: column 3 product, then columns 2, 1, 0.
+        cplx<T> out[1 << R];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+#pragma unroll
+            for (int g = 0; g < (1 << (R - 2)); ++g) {
+                int base = g;
+                base = ((base >> LO) << (LO + 1)) | (base & ((1 << LO) - 1));
+                base = ((base >> HI) << (HI + 1)) | (base & ((1 << HI) - 1));
+                cplx<T> v[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) v[c] = a[base | (((c >> 1) & 1) << P) | ((c & 1) << Q)];
+#pragma unroll
+                for (int r = 2 * half; r < 2 * half + 2; ++r) {
+                    cplx<T> acc;
+                    if (V == V_REAL) {
+                        acc = cplx<T>{m[r * 4 + 3].re * v[3].re, m[r * 4 + 3].re * v[3].im};
+#pragma unroll
+                        for (int c = 2; c >= 0; --c) acc = frmac(acc, m[r * 4 + c], v[c]);
+                    } else {
+                        acc = fmul(m[r * 4 + 3], v[3]);
+#pragma unroll
+                        for (int c = 2; c >= 0; --c) acc = fmac(acc, m[r * 4 + c], v[c]);
+                    }
+                    out[base | (((r >> 1) & 1) << P) | ((r & 1) << Q)] = acc;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < (1 << R); ++j) a[j] = out[j];
+        return;
+    }
 #pragma unroll
     for (int g = 0; g < (1 << (R - 2)); ++g) {
         int base = g;
@@ -219,14 +256,6 @@ __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[1 << R], const cplx<T> *
             cplx<T> acc;
             if (V == V_DIAG) {
                 acc = cmul(m[r * 5], v[r]);
-            } else if (FAST && V == V_REAL) {
-                acc = cplx<T>{m[r * 4 + 3].re * v[3].re, m[r * 4 + 3].re * v[3].im};
-#pragma unroll
-                for (int c = 2; c >= 0; --c) acc = frmac(acc, m[r * 4 + c], v[c]);
-            } else if (FAST) {
-                acc = fmul(m[r * 4 + 3], v[3]);
-#pragma unroll
-                for (int c = 2; c >= 0; --c) acc = fmac(acc, m[r * 4 + c], v[c]);
             } else if (V == V_REAL) {
                 acc = rmul(m[r * 4], v[0]);
 #pragma unroll
