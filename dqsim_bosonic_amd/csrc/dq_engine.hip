@@ -866,10 +866,69 @@ bool reg_path(const dq_state *s) {
     return s->fuse && s->reg_tiles && effective_tile_bits(&cfg) >= 9;
 }
 
-bool all_elementwise(const dq_state *s) {
-    for (const HostOp &h : s->queue)
-        if (h.kind != K_DIAG && h.kind != K_COLLAPSE) return false;
+bool all_elementwise(const dq_state *s, size_t count) {
+    for (size_t i = 0; i < count; ++i)
+        if (s->queue[i].kind != K_DIAG && s->queue[i].kind != K_COLLAPSE) return false;
     return true;
+}
+bool all_elementwise(const dq_state *s) { return all_elementwise(s, s->queue.size()); }
+
+// Whether a queued op is unitary: every entry of U^H U - I within 1e-13 (the
+// C ABI takes any matrix, so this is checked, not assumed).  Selector-fused
+// ops and collapses are not.
+bool queued_unitary(const HostOp &h) {
+    if (h.kind == K_COLLAPSE || h.nsel > 0 || h.k < 1 || h.k > 5) return false;
+    constexpr double tol = 1e-13;
+    const int dim = 1 << h.k;
+    if (h.kind == K_DIAG || h.kind == K_MONO) {
+        if (h.mat.size() != size_t(2 * dim)) return false;
+        for (int r = 0; r < dim; ++r)
+            if (std::fabs(h.mat[2 * r] * h.mat[2 * r] + h.mat[2 * r + 1] * h.mat[2 * r + 1] - 1.0) > tol) return false;
+        if (h.kind == K_DIAG) return true;
+        if (h.perm.size() != size_t(dim)) return false;
+        std::vector<char> seen(dim, 0);
+        for (int c : h.perm) {
+            if (c < 0 || c >= dim || seen[c]) return false;
+            seen[c] = 1;
+        }
+        return true;
+    }
+    const std::vector<double> *u = nullptr;
+    int d = dim;
+    if (h.dense.size() == size_t(2) * dim * dim) {
+        u = &h.dense;
+    } else if ((h.kind == K_DENSE1 || h.kind == K_CTRL1) && h.mat.size() == 8) {
+        u = &h.mat;  // the 2x2 (a controlled 2x2 is unitary when its block is)
+        d = 2;
+    } else if (h.mat.size() == size_t(2) * dim * dim) {
+        u = &h.mat;
+    }
+    if (!u) return false;
+    const double *m = u->data();
+    for (int a = 0; a < d; ++a)
+        for (int b = a; b < d; ++b) {
+            double re = 0.0, im = 0.0;  // column a^H . column b
+            for (int r = 0; r < d; ++r) {
+                const double *x = m + 2 * (size_t(r) * d + a), *y = m + 2 * (size_t(r) * d + b);
+                re += x[0] * y[0] + x[1] * y[1];
+                im += x[0] * y[1] - x[1] * y[0];
+            }
+            if (std::fabs(re - (a == b ? 1.0 : 0.0)) > tol || std::fabs(im) > tol) return false;
+        }
+    return true;
+}
+
+// Queue length whose ops decide p1 of `qubit`: unitaries at the end of the
+// queue that do not act on it leave its marginal unchanged (U acts as I on
+// the qubit, so <P1> is the same before and after), so they are left out.
+size_t marginal_prefix(const dq_state *s, int qubit) {
+    size_t live = s->queue.size();
+    while (live > 0) {
+        const HostOp &h = s->queue[live - 1];
+        if ((((h.qmask() | h.cmask) >> qubit) & 1) || !queued_unitary(h)) break;
+        --live;
+    }
+    return live;
 }
 
 int enqueue(dq_state *s, HostOp &&op) {
@@ -1071,14 +1130,20 @@ int dq_collapse(dq_state *s, int qubit, int outcome, double norm) {
 int dq_prob_one(dq_state *s, int qubit, double *p1) {
     if (!s || !p1) return fail(DQ_ERR_ARG, "null argument");
     if (int rc = check_qubit(s, qubit)) return rc;
-    if (!s->queue.empty() && all_elementwise(s)) {
+    // ops that decide p1 (the queue up to trailing unitaries on other qubits,
+    // such as the x of a reset of another qubit); the rest stay queued
+    const size_t live = marginal_prefix(s, qubit);
+    const bool elementwise = live > 0 && all_elementwise(s, live);
+    if (elementwise) {
         // a pending collapse of this qubit to 0, with only diagonal ops and
         // collapses after it, leaves every amplitude with the bit set at an
-        // exact zero: the sum is +0.0 without reading the state (a reset
-        // right after a measurement that gave 0)
-        for (auto it = s->queue.rbegin(); it != s->queue.rend(); ++it) {
-            if (it->kind != K_COLLAPSE || it->q[0] != qubit) continue;
-            if (it->outcome == 0) {
+        // exact zero (unitaries on other qubits only combine those zeros):
+        // the sum is +0.0 without reading the state (a reset right after a
+        // measurement that gave 0)
+        for (size_t i = live; i-- > 0;) {
+            const HostOp &h = s->queue[i];
+            if (h.kind != K_COLLAPSE || h.q[0] != qubit) continue;
+            if (h.outcome == 0) {
                 *p1 = 0.0;
                 return DQ_OK;
             }
@@ -1088,13 +1153,13 @@ int dq_prob_one(dq_state *s, int qubit, double *p1) {
     const uint64_t count = amp_count(s) / 2;
     const int blocks = std::min<int>(kReduceBlocks, grid_for(count));
     int nparts = blocks;
-    if (!s->queue.empty() && s->queue.size() <= 64 && all_elementwise(s)) {
-        // only diagonals/collapses pending: reduce over the state they would
-        // produce and keep them queued for the next sweep
+    if (elementwise && live <= 64) {
+        // only diagonals/collapses decide p1: reduce over the state they
+        // would produce and keep the whole queue for the next sweep
         Packed pk;
         int pos[64];
         for (int q = 0; q < 64; ++q) pos[q] = q;
-        for (const HostOp &h : s->queue) pk.ops.push_back(to_dev(h, pos, pk));
+        for (size_t i = 0; i < live; ++i) pk.ops.push_back(to_dev(s->queue[i], pos, pk));
         DevOp *d_ops;
         void *d_mats;
         int *d_perms;
@@ -1108,13 +1173,16 @@ int dq_prob_one(dq_state *s, int qubit, double *p1) {
                                static_cast<const cplx<float> *>(s->amps), s->n, qubit, d_ops, int(pk.ops.size()),
                                static_cast<const cplx<float> *>(d_mats), s->d_partials);
         s->launches++;
-    } else if (!s->queue.empty() && reg_path(s)) {
+    } else if (live > 0 && reg_path(s)) {
         // the flush's last sweep sums p1 as it writes the state
         const int rc = s->dtype == DQ_C128 ? flush_t<double>(s, qubit) : flush_t<float>(s, qubit);
         if (rc) return rc;
         nparts = s->last_grid;
     } else {
-        if (int rc = flush(s)) return rc;
+        // nothing queued decides p1 (live == 0): the state in memory has the
+        // same marginal, and the queue stays for the next sweep
+        if (live > 0)
+            if (int rc = flush(s)) return rc;
         if (s->dtype == DQ_C128)
             hipLaunchKernelGGL(k_prob_one<double>, dim3(blocks), dim3(kThreads), 0, s->stream,
                                static_cast<const cplx<double> *>(s->amps), s->n, qubit, s->d_partials);

@@ -260,6 +260,48 @@ def test_prob_one_with_pending_elementwise_ops(n, fuse):
         assert np.max(np.abs(np.array(results[0]) - np.array(results[1]))) <= 1e-15
 
 
+@pytest.mark.parametrize("n", [10, 16])
+@pytest.mark.parametrize("fuse", [1, 2])
+def test_prob_one_leaves_out_trailing_unitaries_on_other_qubits(n, fuse):
+    """After a collapse followed by unitaries that do not act on the measured
+    qubit (the x of a reset of another qubit, say), p1 is reduced without
+    applying those unitaries, which stay queued.  p1 and the final state equal
+    the oracle's with every op applied.  A qubit that a queued unitary does
+    act on, and a non-unitary matrix on another qubit, go through the flush."""
+    rng = np.random.default_rng(70 + n)
+    psi = random_state(n, rng)
+    x, h, cx = orc.gate_matrix("x"), orc.gate_matrix("h"), orc.gate_matrix("cx")
+    shrink = np.array([[1.0, 0.3], [0.2, 0.9]], dtype=np.complex128)  # not unitary
+    tails = {"unitary": [(x, [2]), (h, [n - 1]), (cx, [0, 4])], "non-unitary": [(x, [2]), (shrink, [3])],
+             "unitary, no collapse": [(h, [3]), (cx, [0, 4])]}
+    for tail, ops in tails.items():
+        for q in (5, 2, n - 1):
+            st = capi.State(n)
+            st.set_options(fuse, 0)
+            st.set(psi)
+            ref = psi.copy()
+            if tail != "unitary, no collapse":
+                p2 = orc.prob_one(ref, n, 2)
+                st.collapse(2, 1, max(np.sqrt(p2), 1e-15))
+                orc.collapse(ref, n, 2, 1, p2)
+            for u, qs in ops:
+                if len(qs) == 1:
+                    st.apply_1q(u, qs[0])
+                    orc.apply_one_qubit(ref, n, u, qs[0])
+                else:
+                    st.apply_nq(u, qs)
+                    orc.apply_n_qubit(ref, n, u, qs)
+            p = st.prob_one(q)
+            p_ref = orc.prob_one(ref, n, q)
+            assert abs(p - p_ref) <= 1e-14 * max(1.0, abs(p_ref)), (tail, q, p, p_ref)
+            out = st.get()
+            st.close()
+            if fuse == 1:
+                assert np.array_equal(out, ref), (tail, q)
+            else:
+                assert np.max(np.abs(out - ref)) <= 1e-13, (tail, q)
+
+
 @pytest.mark.parametrize("fuse", [0, 1, 2])
 def test_prob_one_fused_into_sweep(fuse):
     """p1 summed by the last sweep of a flush agrees with the oracle."""
