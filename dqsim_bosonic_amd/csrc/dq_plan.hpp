@@ -27,6 +27,7 @@ struct PlanConfig {
     double fuse_min_cost = 128.0;  // fusion level 3: least cost of a fused block's gates
     int pair_fusion = 1;           // fusion levels 2-3: merge neighbouring gates on <= 2 qubits
     int reg_bits = 4;              // register bits of a register-tile layout (4: 16, 3: 8 amplitudes per thread)
+    int phase_forms = 1;           // fusion levels 2-3: 4x4 ops factored as U = D R or R D (V_ROWPH / V_COLPH)
 };
 
 inline bool dq_tile_bits_supported(int b) { return b == 9 || b == 12 || b == 13; }
@@ -622,6 +623,60 @@ inline int op_variant(const std::vector<double> &m, int dim) {
     return V_GEN;
 }
 
+// Phase-factored forms of 4x4 matrices (V_ROWPH / V_COLPH; see OpVariant).
+// `m` holds one or more 4x4 matrices (the selector blocks of an op).  A row
+// (column) qualifies when every entry, turned by the phase of its largest
+// entry, has an imaginary part within 4e-15 of that entry's magnitude, the
+// rounding of a product of a few gate matrices; the dropped imaginary parts
+// are of that order.  Returns the variant and rewrites `m` into its encoding,
+// or returns V_GEN and leaves `m` alone.
+inline int phase_factor4(std::vector<double> &m) {
+    const size_t nblk = m.size() / 32;
+    if (nblk == 0 || m.size() % 32) return V_GEN;
+    for (const int form : {V_ROWPH, V_COLPH}) {
+        std::vector<double> enc(m.size(), 0.0);
+        bool ok = true;
+        for (size_t b = 0; b < nblk && ok; ++b) {
+            const double *u = m.data() + 32 * b;
+            double *e = enc.data() + 32 * b;
+            for (int k = 0; k < 4 && ok; ++k) {
+                // entry j of row k (V_ROWPH) or of column k (V_COLPH)
+                auto at = [&](int j) { return form == V_ROWPH ? 4 * k + j : 4 * j + k; };
+                int jm = 0;
+                double am = -1.0;
+                for (int j = 0; j < 4; ++j) {
+                    const double a = std::hypot(u[2 * at(j)], u[2 * at(j) + 1]);
+                    if (a > am) {
+                        am = a;
+                        jm = j;
+                    }
+                }
+                if (!(am > 0.0)) {
+                    ok = false;
+                    break;
+                }
+                const double pr = u[2 * at(jm)] / am, pi = u[2 * at(jm) + 1] / am;
+                for (int j = 0; j < 4; ++j) {
+                    const double ur = u[2 * at(j)], ui = u[2 * at(j) + 1];
+                    const double wr = ur * pr + ui * pi, wi = ui * pr - ur * pi;  // u * conj(phase)
+                    if (std::fabs(wi) > 4e-15 * am) {
+                        ok = false;
+                        break;
+                    }
+                    e[2 * at(j)] = wr;
+                }
+                e[2 * (4 * k) + 1] = pr;
+                e[2 * (4 * k + 1) + 1] = pi;
+            }
+        }
+        if (ok) {
+            m.swap(enc);
+            return form;
+        }
+    }
+    return V_GEN;
+}
+
 inline std::vector<double> diag_to_dense(const std::vector<double> &d) {
     const size_t dim = d.size() / 2;
     std::vector<double> m(2 * dim * dim, 0.0);
@@ -960,8 +1015,17 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
             break;
         }
         t.mat = int(mats.size() / 2);
-        mats.insert(mats.end(), m->begin(), m->end());
         if (t.kind == T_DENSE1 || t.kind == T_DENSE2) t.variant = op_variant(*m, t.kind == T_DENSE1 ? 2 : 4);
+        if (t.kind == T_DENSE2 && t.variant == V_GEN && s->fuse >= 2 && s->phase_forms) {
+            // contracted kernels only: U = D R or R D as a real op and phases
+            if (m != &tmp) tmp = *m;
+            const int v = phase_factor4(tmp);
+            if (v != V_GEN) {
+                t.variant = v;
+                m = &tmp;
+            }
+        }
+        mats.insert(mats.end(), m->begin(), m->end());
         if (t.kind == T_CTRL1) t.variant = op_variant(*m, 2) == V_ANTI ? V_ANTI : V_GEN;
         if ((t.kind == T_DENSE1 || t.kind == T_DENSE2) && h.nsel > 0) {
             t.nsel = h.nsel;

@@ -212,6 +212,22 @@ __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[1 << R], const cplx<T> *
         // once; with all four rows live the scalar registers spill and every
         // op reloads them (10-16 v_readlane per op).  Per output the FMA
         // chain is unchanged: column 3 product, then columns 2, 1, 0.
+        // V_ROWPH / V_COLPH: real rows (the real parts of the entries), with
+        // each output (V_ROWPH) or each input (V_COLPH) times its phase.
+        constexpr bool kReal = V == V_REAL || V == V_ROWPH || V == V_COLPH;
+        if constexpr (V == V_COLPH) {
+#pragma unroll
+            for (int g = 0; g < (1 << (R - 2)); ++g) {
+                int base = g;
+                base = ((base >> LO) << (LO + 1)) | (base & ((1 << LO) - 1));
+                base = ((base >> HI) << (HI + 1)) | (base & ((1 << HI) - 1));
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int j = base | (((c >> 1) & 1) << P) | ((c & 1) << Q);
+                    a[j] = fmul(cplx<T>{m[4 * c].im, m[4 * c + 1].im}, a[j]);
+                }
+            }
+        }
         cplx<T> out[1 << R];
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
@@ -226,10 +242,11 @@ __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[1 << R], const cplx<T> *
 #pragma unroll
                 for (int r = 2 * half; r < 2 * half + 2; ++r) {
                     cplx<T> acc;
-                    if (V == V_REAL) {
+                    if (kReal) {
                         acc = cplx<T>{m[r * 4 + 3].re * v[3].re, m[r * 4 + 3].re * v[3].im};
 #pragma unroll
                         for (int c = 2; c >= 0; --c) acc = frmac(acc, m[r * 4 + c], v[c]);
+                        if (V == V_ROWPH) acc = fmul(cplx<T>{m[4 * r].im, m[4 * r + 1].im}, acc);
                     } else {
                         acc = fmul(m[r * 4 + 3], v[3]);
 #pragma unroll
@@ -251,15 +268,20 @@ __device__ __forceinline__ void reg_dense2(cplx<T> (&a)[1 << R], const cplx<T> *
         cplx<T> v[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[c] = a[base | (((c >> 1) & 1) << P) | ((c & 1) << Q)];
+        if (V == V_COLPH) {  // (the host sends the phase forms to contracted kernels only)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = cmul(cplx<T>{m[4 * c].im, m[4 * c + 1].im}, v[c]);
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             cplx<T> acc;
             if (V == V_DIAG) {
                 acc = cmul(m[r * 5], v[r]);
-            } else if (V == V_REAL) {
+            } else if (V == V_REAL || V == V_ROWPH || V == V_COLPH) {
                 acc = rmul(m[r * 4], v[0]);
 #pragma unroll
                 for (int c = 1; c < 4; ++c) acc = cadd(acc, rmul(m[r * 4 + c], v[c]));
+                if (V == V_ROWPH) acc = cmul(cplx<T>{m[4 * r].im, m[4 * r + 1].im}, acc);
             } else {
                 acc = cmul(m[r * 4], v[0]);
 #pragma unroll
@@ -324,6 +346,8 @@ __device__ __forceinline__ void dense2_v(cplx<T> (&a)[1 << R], int v, const cplx
     switch (v) {
     case V_REAL: reg_dense2<T, P, Q, V_REAL, FAST, R>(a, m); break;
     case V_DIAG: reg_dense2<T, P, Q, V_DIAG, FAST, R>(a, m); break;
+    case V_ROWPH: reg_dense2<T, P, Q, V_ROWPH, FAST, R>(a, m); break;
+    case V_COLPH: reg_dense2<T, P, Q, V_COLPH, FAST, R>(a, m); break;
     default: reg_dense2<T, P, Q, V_GEN, FAST, R>(a, m); break;
     }
 }

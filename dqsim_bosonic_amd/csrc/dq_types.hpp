@@ -43,10 +43,16 @@ enum TileKind : int {
                      // real 64x64 GEMM on FP32 MFMA over the tile's 32-amplitude columns
 };
 
-// Matrix shape of a register op, chosen on the host.  Every variant computes
-// the reference's row sums term by term; V_REAL drops only the products of
-// zero imaginary parts, V_DIAG / V_ANTI only the products of zero entries.
-enum OpVariant : int { V_GEN = 0, V_REAL = 1, V_DIAG = 2, V_ANTI = 3 };
+// Matrix shape of a register op, chosen on the host.  V_GEN .. V_ANTI
+// compute the reference's row sums term by term; V_REAL drops only the
+// products of zero imaginary parts, V_DIAG / V_ANTI only the products of
+// zero entries.  V_ROWPH and V_COLPH (4x4 only, contracted kernels only,
+// i.e. fused ops of fusion levels 2-3) factor U = D R (each row a real row
+// times one phase) or U = R D (each column): the op is a real 4x4 plus one
+// complex product per amplitude, 12 FP64 instructions per amplitude instead
+// of 16.  Their matrix slot holds R in the real parts of the 16 entries and
+// phase k in the imaginary parts of entries 4k and 4k + 1.
+enum OpVariant : int { V_GEN = 0, V_REAL = 1, V_DIAG = 2, V_ANTI = 3, V_ROWPH = 4, V_COLPH = 5 };
 
 struct TileOp {
     // dispatch header: the first 12 words, read as one block by the kernel

@@ -436,6 +436,13 @@ PYBIND11_MODULE(_core, m) {
         return d;
     }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
        py::arg("reg_bits") = 4);
+    // The phase-factored form of 4x4 matrices (one or more, as (re, im)
+    // pairs, row-major): (variant, encoded entries), the entries unchanged
+    // when the variant is V_GEN.
+    m.def("_phase_factor4", [](std::vector<double> mat) {
+        const int v = dq::phase_factor4(mat);
+        return py::make_tuple(v, mat);
+    });
     // The register-tile plan of an op list, pass by pass: tile bits, load and
     // store layouts, and each tile op as (kind, variant, relayout, layout,
     // register slots p and q, tile bits).

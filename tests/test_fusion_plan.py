@@ -221,3 +221,79 @@ def test_pairs_then_blocks_equal_program(seed):
             qs, mats, _, sq = item
             got = apply_selected(got, n, qs, mats, sq)
     assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def _decode_phase_form(v, enc):
+    """The 4x4 matrix that a V_ROWPH (4) / V_COLPH (5) encoding applies: R from
+    the real parts, phase k from the imaginary parts of entries 4k and 4k + 1."""
+    e = np.asarray(enc, dtype=np.float64).reshape(16, 2)
+    r = e[:, 0].reshape(4, 4)
+    ph = np.array([e[4 * k, 1] + 1j * e[4 * k + 1, 1] for k in range(4)])
+    return ph[:, None] * r if v == 4 else r * ph[None, :]
+
+
+def _flat(u):
+    return np.stack([u.real, u.imag], axis=-1).ravel().tolist()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_phase_forms_factor_and_reproduce_the_matrix(seed):
+    """U = D R (rows with one phase each) is V_ROWPH, U = R D is V_COLPH, and
+    the encoding reproduces U to rounding; a generic unitary stays V_GEN.  A
+    wrong phase index, a transposed factor or a dropped conjugate fails by
+    O(1)."""
+    rng = np.random.default_rng(seed)
+    rq, _ = np.linalg.qr(rng.normal(size=(4, 4)))  # real orthogonal
+    d = np.exp(1j * rng.uniform(-np.pi, np.pi, size=4))
+    for u, want in ((d[:, None] * rq, 4), (rq * d[None, :], 5)):
+        v, enc = core._phase_factor4(_flat(u))
+        assert v == want
+        assert np.max(np.abs(_decode_phase_form(v, enc) - u)) <= 4e-16
+    g, _ = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))
+    v, enc = core._phase_factor4(_flat(g))
+    assert v == 0 and np.array_equal(np.asarray(enc), np.asarray(_flat(g)))
+    # the HEA step RZ_q RY_q CX(q, q+1) is a diagonal times a real matrix
+    ry = orc.gate_matrix("ry", [rng.uniform(-4, 4)])
+    rz = orc.gate_matrix("rz", [rng.uniform(-4, 4)])
+    u = np.kron(rz @ ry, np.eye(2)) @ orc.gate_matrix("cx", [])
+    v, enc = core._phase_factor4(_flat(u))
+    assert v == 4 and np.max(np.abs(_decode_phase_form(v, enc) - u)) <= 4e-16
+
+
+def test_phase_forms_several_selector_blocks_and_a_near_miss():
+    """Every selector block must factor (then each is encoded); an imaginary
+    part above the 4e-15 tolerance in one row keeps the op general."""
+    rng = np.random.default_rng(11)
+    blocks = []
+    for _ in range(2):
+        rq, _ = np.linalg.qr(rng.normal(size=(4, 4)))
+        blocks.append(np.exp(1j * rng.uniform(-3, 3, size=4))[:, None] * rq)
+    v, enc = core._phase_factor4(_flat(blocks[0]) + _flat(blocks[1]))
+    assert v == 4
+    for b in range(2):
+        assert np.max(np.abs(_decode_phase_form(v, enc[32 * b:32 * b + 32]) - blocks[b])) <= 4e-16
+    bad = blocks[1].copy()
+    bad[2, 1] *= np.exp(1j * 1e-9)
+    v, _ = core._phase_factor4(_flat(blocks[0]) + _flat(bad))
+    assert v == 0
+
+
+def test_phase_forms_only_with_pair_fusion():
+    """The HEA family's fused 4x4 ops take the row-phase form at fusion 2;
+    fusion 1 (the exact kernels, reference arithmetic) never uses it."""
+    rng = np.random.default_rng(3)
+    n = 14
+    ops = []
+    for _ in range(4):
+        for q in range(n):
+            ops.append(([q], orc.gate_matrix("ry", [rng.uniform(0, 6)]).ravel(), []))
+        for q in range(n):
+            ops.append(([q], orc.gate_matrix("rz", [rng.uniform(0, 6)]).ravel(), []))
+        for q in range(n - 1):
+            ops.append(([q, q + 1], orc.gate_matrix("cx", []).ravel(), []))
+    variants = {}
+    for fuse in (1, 2):
+        vs = [o[1] for ps in core._plan_passes(n, ops, fuse, 12, 4, 4) for o in ps["ops"] if o[0] == 2]
+        variants[fuse] = vs
+    assert variants[2].count(4) >= len(variants[2]) // 2
+    assert 4 not in variants[1] and 5 not in variants[1]

@@ -302,6 +302,52 @@ def test_prob_one_leaves_out_trailing_unitaries_on_other_qubits(n, fuse):
                 assert np.max(np.abs(out - ref)) <= 1e-13, (tail, q)
 
 
+@pytest.mark.parametrize("reg_bits", [4, 3])
+def test_phase_form_ops_match_oracle(reg_bits):
+    """Fused 4x4 ops of the forms U = D R (ry, rz then cx chains) and U = R D
+    (cp or rzz followed by h / ry on both qubits) run as a real 4x4 plus
+    phases in the contracted kernels; the state equals the oracle's to 1e-12.
+    The plan is checked to contain both forms, so the test exercises them."""
+    from dqsim_bosonic_amd import _core
+
+    n = 17
+    rng = np.random.default_rng(90 + reg_bits)
+    psi = random_state(n, rng)
+    prog = []
+    for layer in range(5):
+        for q in range(n):
+            prog.append((orc.gate_matrix("ry", [rng.uniform(0, 6)]), [q]))
+        for q in range(n):
+            prog.append((orc.gate_matrix("rz", [rng.uniform(0, 6)]), [q]))
+        for q in range(n - 1):
+            prog.append((orc.gate_matrix("cx"), [q, q + 1]))
+        perm = [int(x) for x in rng.permutation(n)]
+        for i in range(0, n - 1, 2):
+            a, b = perm[i], perm[i + 1]
+            d = orc.gate_matrix("cp", [rng.uniform(0, 6)]) if i % 4 else orc.gate_matrix("rzz", [rng.uniform(0, 6)])
+            prog.append((d, [a, b]))
+            prog.append((orc.gate_matrix("h"), [a]))
+            prog.append((orc.gate_matrix("ry", [rng.uniform(0, 6)]), [b]))
+    plan = _core._plan_passes(n, [(qs, u.ravel(), []) for u, qs in prog], 2, 12, 4, reg_bits)
+    variants = {o[1] for ps in plan for o in ps["ops"] if o[0] == 2}
+    assert {4, 5} <= variants, variants
+    st = capi.State(n)
+    st.set_options(2, 0)
+    st.set_reg_bits(reg_bits)
+    st.set(psi)
+    ref = psi.copy()
+    for u, qs in prog:
+        if len(qs) == 1:
+            st.apply_1q(u, qs[0])
+            orc.apply_one_qubit(ref, n, u, qs[0])
+        else:
+            st.apply_nq(u, qs)
+            orc.apply_n_qubit(ref, n, u, qs)
+    out = st.get()
+    st.close()
+    assert np.max(np.abs(out - ref)) <= 1e-12
+
+
 @pytest.mark.parametrize("fuse", [0, 1, 2])
 def test_prob_one_fused_into_sweep(fuse):
     """p1 summed by the last sweep of a flush agrees with the oracle."""
