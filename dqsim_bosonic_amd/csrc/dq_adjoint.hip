@@ -310,9 +310,13 @@ enum AdjRKind : int {
                     // gradient g in its term's `slot`; b = 1: then the phase table (mask, mat) of R_DTABLE
     R_GRAD1R = 9,   // R_GRAD1 on slot a (b = 1 for Y, term term0, partial slot `slot`) followed by
                     // R_DENSE1R on the same slot (entries at mat): a rotation's gradient and its inverse
-    R_ROT3 = 10     // three R_GRAD1R, on register slots 0, 1, 2 in that order, entry term0 + m for slot
+    R_ROT3 = 10,    // three R_GRAD1R, on register slots 0, 1, 2 in that order, entry term0 + m for slot
                     // m: 1 for Y in `z`, the real 2x2 at mats offset `xt`, partial slot `slot` (-1: no
                     // gradient; the identity and a zero coefficient stand in for an absent rotation)
+    R_PERM2 = 11,   // a 4x4 of exact zeros and ones that only exchanges amplitudes: k = 0, cx (control
+                    // slot a = the matrix MSB, target slot b); k = 1, exchange of slots a < b (swap)
+    R_CTRL2 = 12    // a 4x4 that is the identity where the matrix MSB (slot a) is 0: the 2x2 block (at
+                    // mats offset `mat`, 4 entries) on target slot b where it is 1 (crx, cry, cu3, ...)
 };
 
 struct AdjRStep {
@@ -531,6 +535,45 @@ __device__ __forceinline__ void r_dense2(cplx<T> (&a)[NR], const cplx<T> (&u)[16
             a[idx[r]] = acc;
         }
     }
+}
+
+// Register permutations (R_PERM2): pure exchanges of amplitudes, no arithmetic.
+// r_cswap: where register bit C is set, exchange across register bit X (cx).
+template <typename T, int NR, int C, int X>
+__device__ __forceinline__ void r_cswap(cplx<T> (&a)[NR]) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+        if (((j >> C) & 1) && !((j >> X) & 1)) {
+            const cplx<T> t = a[j];
+            a[j] = a[j | (1 << X)];
+            a[j | (1 << X)] = t;
+        }
+}
+// r_swap2: exchange register bits A and B (swap).
+template <typename T, int NR, int A, int B>
+__device__ __forceinline__ void r_swap2(cplx<T> (&a)[NR]) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+        if (((j >> A) & 1) && !((j >> B) & 1)) {
+            const int k = j ^ (1 << A) ^ (1 << B);
+            const cplx<T> t = a[j];
+            a[j] = a[k];
+            a[k] = t;
+        }
+}
+
+// Controlled 2x2 (R_CTRL2): where register bit C is set, the pair across
+// register bit X takes u (row-major 2x2); the products run in the order of the
+// 4x4's row sums (column 3, then 2), whose other two terms are exact zeros.
+template <typename T, int NR, int C, int X>
+__device__ __forceinline__ void r_ctrl2(cplx<T> (&a)[NR], const cplx<T> (&u)[4]) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+        if (((j >> C) & 1) && !((j >> X) & 1)) {
+            const cplx<T> x = a[j], y = a[j | (1 << X)];
+            a[j] = cfmac(cfmul(u[1], y), u[0], x);
+            a[j | (1 << X)] = cfmac(cfmul(u[3], y), u[2], x);
+        }
 }
 
 // sum over this thread's amplitudes of Im(conj(l) * (G p)) for one term set
@@ -794,6 +837,33 @@ __global__ __launch_bounds__(1 << (B - RB), sizeof(T) == 4 ? 4 : 2) void k_adjoi
                 case 6: r_dense2<T, NR, 1 % RB, 2 % RB>(p, u); r_dense2<T, NR, 1 % RB, 2 % RB>(l, u); break;
                 case 8: r_dense2<T, NR, 2 % RB, 0>(p, u); r_dense2<T, NR, 2 % RB, 0>(l, u); break;
                 default: r_dense2<T, NR, 2 % RB, 1 % RB>(p, u); r_dense2<T, NR, 2 % RB, 1 % RB>(l, u); break;
+                }
+                break;
+            }
+            case R_PERM2:
+                switch (st.k * 16 + st.a * 4 + st.b) {
+                case 1: r_cswap<T, NR, 0, 1 % RB>(p); r_cswap<T, NR, 0, 1 % RB>(l); break;
+                case 2: r_cswap<T, NR, 0, 2 % RB>(p); r_cswap<T, NR, 0, 2 % RB>(l); break;
+                case 4: r_cswap<T, NR, 1 % RB, 0>(p); r_cswap<T, NR, 1 % RB, 0>(l); break;
+                case 6: r_cswap<T, NR, 1 % RB, 2 % RB>(p); r_cswap<T, NR, 1 % RB, 2 % RB>(l); break;
+                case 8: r_cswap<T, NR, 2 % RB, 0>(p); r_cswap<T, NR, 2 % RB, 0>(l); break;
+                case 9: r_cswap<T, NR, 2 % RB, 1 % RB>(p); r_cswap<T, NR, 2 % RB, 1 % RB>(l); break;
+                case 17: r_swap2<T, NR, 0, 1 % RB>(p); r_swap2<T, NR, 0, 1 % RB>(l); break;
+                case 18: r_swap2<T, NR, 0, 2 % RB>(p); r_swap2<T, NR, 0, 2 % RB>(l); break;
+                default: r_swap2<T, NR, 1 % RB, 2 % RB>(p); r_swap2<T, NR, 1 % RB, 2 % RB>(l); break;  // 22
+                }
+                break;
+            case R_CTRL2: {
+                cplx<T> u[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) u[e] = cvt<T>(mats[st.mat + e]);
+                switch (st.a * 4 + st.b) {
+                case 1: r_ctrl2<T, NR, 0, 1 % RB>(p, u); r_ctrl2<T, NR, 0, 1 % RB>(l, u); break;
+                case 2: r_ctrl2<T, NR, 0, 2 % RB>(p, u); r_ctrl2<T, NR, 0, 2 % RB>(l, u); break;
+                case 4: r_ctrl2<T, NR, 1 % RB, 0>(p, u); r_ctrl2<T, NR, 1 % RB, 0>(l, u); break;
+                case 6: r_ctrl2<T, NR, 1 % RB, 2 % RB>(p, u); r_ctrl2<T, NR, 1 % RB, 2 % RB>(l, u); break;
+                case 8: r_ctrl2<T, NR, 2 % RB, 0>(p, u); r_ctrl2<T, NR, 2 % RB, 0>(l, u); break;
+                default: r_ctrl2<T, NR, 2 % RB, 1 % RB>(p, u); r_ctrl2<T, NR, 2 % RB, 1 % RB>(l, u); break;
                 }
                 break;
             }
@@ -1095,11 +1165,62 @@ bool to_register_steps(const AdjStep *st, int nsteps, const std::vector<AdjTerm>
             if (real) r.kind = R_DENSE1R;
             break;
         }
-        case A_DENSE2:
+        case A_DENSE2: {
             r.kind = R_DENSE2;
             r.a = slot_of(mask, x.bit[0]);
             r.b = slot_of(mask, x.bit[1]);
+            // exact permutations of the cx / swap pattern exchange registers
+            // (entries exactly 0 or 1): src[row] = the column of row's 1
+            int src[4];
+            bool perm = true;
+            for (int row = 0; row < 4 && perm; ++row) {
+                src[row] = -1;
+                for (int col = 0; col < 4; ++col) {
+                    const cplx<double> e = hmats[size_t(x.mat + 4 * row + col)];
+                    if (e.im != 0.0 || (e.re != 0.0 && e.re != 1.0)) perm = false;
+                    if (e.re == 1.0) {
+                        if (src[row] >= 0) perm = false;
+                        src[row] = col;
+                    }
+                }
+                if (src[row] < 0) perm = false;
+            }
+            if (perm) {
+                const int code = src[0] * 64 + src[1] * 16 + src[2] * 4 + src[3];
+                if (code == 0 * 64 + 1 * 16 + 3 * 4 + 2) {  // rows 2, 3 exchanged: cx, control = MSB
+                    r.kind = R_PERM2;
+                    r.k = 0;
+                } else if (code == 0 * 64 + 2 * 16 + 1 * 4 + 3) {  // rows 1, 2 exchanged: swap
+                    r.kind = R_PERM2;
+                    r.k = 1;
+                    const int lo = std::min(r.a, r.b), hi = std::max(r.a, r.b);
+                    r.a = lo;
+                    r.b = hi;
+                }
+                break;
+            }
+            // identity where the MSB is 0 (rows and columns 0, 1), no
+            // coupling between the halves: a controlled 2x2 on the LSB
+            bool ctrl = true;
+            for (int row = 0; row < 4 && ctrl; ++row)
+                for (int col = 0; col < 4; ++col) {
+                    if (row >= 2 && col >= 2) continue;
+                    const cplx<double> e = hmats[size_t(x.mat + 4 * row + col)];
+                    const double want = row == col ? 1.0 : 0.0;
+                    if (e.re != want || e.im != 0.0) {
+                        ctrl = false;
+                        break;
+                    }
+                }
+            if (ctrl) {
+                r.kind = R_CTRL2;
+                const cplx<double> blk[4] = {hmats[size_t(x.mat + 10)], hmats[size_t(x.mat + 11)],
+                                             hmats[size_t(x.mat + 14)], hmats[size_t(x.mat + 15)]};
+                r.mat = int(hmats.size());
+                hmats.insert(hmats.end(), blk, blk + 4);
+            }
             break;
+        }
         case A_DIAG:
             r.kind = R_DIAG;
             r.k = x.k;

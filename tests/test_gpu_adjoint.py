@@ -202,6 +202,43 @@ def test_layered_ansatz_grouped_steps(n, layers):
     assert np.max(np.abs(g32 - g_t)) <= 1e-4 * max(1.0, np.max(np.abs(g_t)))
 
 
+@pytest.mark.parametrize("n", [11, 13])
+def test_register_exchange_and_controlled_steps(n):
+    """cx ladders and swaps (their inverses exchange registers, no arithmetic)
+    and fixed controlled rotations (their inverses touch only the control-1
+    half) in the reverse pass: gradients equal the per-parameter sweep to
+    1e-12, the sweep ends at |0...0>, complex64 agrees to 1e-4 relative."""
+    rng = np.random.default_rng(200 + n)
+    pc = A.ParametricCircuit(n)
+    k = 0
+    for layer in range(3):
+        for q in range(n):
+            pc.ry(q, param=k); k += 1
+            pc.rz(q, param=k); k += 1
+        for q in range(n - 1):
+            pc.cx(q, q + 1)
+        pc.cx(n - 1, 0)
+        for q in range(layer % 2, n - 1, 3):
+            pc.swap(q, q + 1)
+        for q in range(0, n - 1, 2):
+            pc.crx(q + 1, q, value=float(rng.uniform(0, 6)))
+            pc.cry(q, q + 1, value=float(rng.uniform(0, 6)))
+        pc.ch(n - 1, 1)
+    H = A.PauliSum.zz_chain(n) + A.PauliSum.x_sum(n, 0.7)
+    th = rng.uniform(0.0, 2 * math.pi, pc.num_params)
+    est = A.Estimator(pc, H)
+    e_t, g_t = est.gradient(th)
+    assert abs(abs(est._eng.state().statevector[0]) - 1.0) <= 1e-12
+    est._eng.set_tiled_sweep(False)
+    e_p, g_p = est.gradient(th)
+    assert abs(e_t - e_p) <= 1e-13
+    assert np.max(np.abs(g_t - g_p)) <= 1e-12
+    psi = oracle_state(pc, th)
+    assert abs(e_t - float(np.real(np.vdot(psi, dense_h(n, H) @ psi)))) <= 1e-12
+    _, g32 = A.Estimator(pc, H, dtype="c64").gradient(th)
+    assert np.max(np.abs(g32 - g_t)) <= 1e-4 * max(1.0, np.max(np.abs(g_t)))
+
+
 def test_tiled_reverse_sweep_against_parameter_shift():
     pc, H, th = wide_circuit(10, 3, layers=2)
     est = A.Estimator(pc, H)
