@@ -520,7 +520,8 @@ def main() -> int:
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(n),
                 "traffic_source": f"profiles/tile_pass_{n}q_traffic.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE of this "
                                   "kernel from a separate counter run (not measured in this run)",
-                "kernel": "k_tile_reg8<double,12>" if args.reg_bits == 3 else "k_tile_reg<double,12>", "avg_launch_ms": round(avg_launch_ms, 4),
+                "kernel": f"k_tile_reg{'8' if args.reg_bits == 3 else ''}<double,{args.tile_bits or 12}>",
+                "avg_launch_ms": round(avg_launch_ms, 4),
                 "launches_timed": int(timed), "sweeps_per_step": sweeps_per_step,
                 "kernel_time_frac_of_step": round(kernel_ms / 1e3 / elapsed, 4)}
 
