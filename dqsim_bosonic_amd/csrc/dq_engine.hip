@@ -308,10 +308,16 @@ __global__ __launch_bounds__(kThreads) void k_apply_pauli_sum(cplx<T> *__restric
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < count; r += stride) {
         double ar = 0.0, ai = 0.0;
+        uint64_t y = r;
+        cplx<T> v{};
         for (int k = 0; k < nterms; ++k) {
             const PauliTerm t = terms[k];
-            const uint64_t y = r ^ t.x;
-            const cplx<T> v = src[y];
+            // consecutive terms with the same X part share the partner
+            // amplitude (a Z-string run: one load for the whole run)
+            if (k == 0 || t.x != terms[k - 1].x) {
+                y = r ^ t.x;
+                v = src[y];
+            }
             const double sgn = (__popcll(t.z & y) & 1) ? -1.0 : 1.0;
             const double vr = sgn * double(v.re), vi = sgn * double(v.im);
             ar += t.cre * vr - t.cim * vi;
@@ -332,10 +338,14 @@ __global__ __launch_bounds__(kThreads) void k_pauli_inner(const cplx<T> *__restr
     for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < count; r += stride) {
         const double br = double(bra[r].re), bi = -double(bra[r].im);
         double ar = 0.0, ai = 0.0;
+        uint64_t y = r;
+        cplx<T> v{};
         for (int k = 0; k < nterms; ++k) {
             const PauliTerm t = terms[k];
-            const uint64_t y = r ^ t.x;
-            const cplx<T> v = ket[y];
+            if (k == 0 || t.x != terms[k - 1].x) {  // as in k_apply_pauli_sum
+                y = r ^ t.x;
+                v = ket[y];
+            }
             const double sgn = (__popcll(t.z & y) & 1) ? -1.0 : 1.0;
             const double vr = sgn * double(v.re), vi = sgn * double(v.im);
             ar += t.cre * vr - t.cim * vi;
