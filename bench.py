@@ -13,6 +13,11 @@ ChaCha8 draws, resets, conditional); the circuit is decoded once, before
 timing.  value = gates applied per second over the K timed steps (a
 conditional gate counts when its condition held).
 
+draw_margins: the distance of each measurement's ChaCha8 draw from its p1.
+p1 is a reduction whose summation order differs from the reference's (whose
+own order is not fixed), so an outcome could differ from the reference's only
+for a draw within rounding of p1; near_ties counts draws within 1e-12.
+
 roofline: the dominant kernel is the LDS-tiled sweep (k_tile_reg); its
 launches are bracketed by HIP events on the engine's stream.  Algorithmic
 bytes per launch = one read + one write of every amplitude = 2 * 16 * 2^n.
@@ -497,6 +502,16 @@ def main() -> int:
     sweeps1, launches1, _ = eng.stats()
     eng.set_timing(False)
     assert all(r == records[0] for r in records), "measurement record changed between steps"
+    # Distance of each measurement's draw from its p1 (one ChaCha8 draw per
+    # measure and reset, in program order: (next_u64 >> 11) 2^-53).  p1 is a
+    # sum whose order differs from the reference's (its own order is not
+    # fixed either); an outcome could differ only for a draw within rounding
+    # of p1, so the record is exact when no draw lies within 1e-12 of it.
+    p1s = list(eng.p1_record())
+    draws = [(u >> 11) * 2.0 ** -53 for u in _core._rng_stream(args.seed, len(p1s))]
+    gaps = [abs(r - p) for r, p in zip(draws, p1s)]
+    margins = {"measurements_per_step": len(p1s), "min_abs_draw_minus_p1": min(gaps) if gaps else None,
+               "near_ties_below_1e-12": sum(1 for g in gaps if g < 1e-12)}
 
     elapsed = t1 - t0
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -579,6 +594,7 @@ def main() -> int:
                    "seed": args.seed,
                    "fusion": args.fusion, "reg_bits": args.reg_bits,
                    "measurement_record": records[0]},
+        "draw_margins": margins,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adj,
