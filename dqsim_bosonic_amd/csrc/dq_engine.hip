@@ -198,7 +198,22 @@ __global__ __launch_bounds__(kThreads) void k_prob_one(const cplx<T> *__restrict
     const uint64_t count = uint64_t(1) << (n - 1);
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     double s = 0.0;
-    for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < count; g += stride) {
+    uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    // four loads in flight per thread, summed in the plain loop's order
+    for (; g + 3 * stride < count; g += 4 * stride) {
+        cplx<T> a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t gu = g + uint64_t(u) * stride;
+            a[u] = amps[((gu >> q) << (q + 1)) | half | (gu & (half - 1))];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double re = double(a[u].re), im = double(a[u].im);
+            s += re * re + im * im;
+        }
+    }
+    for (; g < count; g += stride) {
         const uint64_t i = ((g >> q) << (q + 1)) | half | (g & (half - 1));
         const cplx<T> a = amps[i];
         const double re = double(a.re), im = double(a.im);
@@ -221,10 +236,7 @@ __global__ __launch_bounds__(kThreads) void k_prob_one_virtual(const cplx<T> *__
     const uint64_t half = uint64_t(1) << q;
     const uint64_t count = uint64_t(1) << (n - 1);
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    double s = 0.0;
-    for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < count; g += stride) {
-        const uint64_t i = ((g >> q) << (q + 1)) | half | (g & (half - 1));
-        cplx<T> a = amps[i];
+    auto apply = [&](uint64_t i, cplx<T> a) {
         for (int o = 0; o < nops; ++o) {
             const DevOp &op = ops[o];
             if (op.kind == K_DIAG) {
@@ -238,6 +250,30 @@ __global__ __launch_bounds__(kThreads) void k_prob_one_virtual(const cplx<T> *__
                 }
             }
         }
+        return a;
+    };
+    double s = 0.0;
+    uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    // as in k_prob_one: four loads in flight, summed in the plain loop's order
+    for (; g + 3 * stride < count; g += 4 * stride) {
+        uint64_t iu[4];
+        cplx<T> a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t gu = g + uint64_t(u) * stride;
+            iu[u] = ((gu >> q) << (q + 1)) | half | (gu & (half - 1));
+            a[u] = amps[iu[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const cplx<T> b = apply(iu[u], a[u]);
+            const double re = double(b.re), im = double(b.im);
+            s += re * re + im * im;
+        }
+    }
+    for (; g < count; g += stride) {
+        const uint64_t i = ((g >> q) << (q + 1)) | half | (g & (half - 1));
+        const cplx<T> a = apply(i, amps[i]);
         const double re = double(a.re), im = double(a.im);
         s += re * re + im * im;
     }
