@@ -920,8 +920,8 @@ bool all_elementwise(const dq_state *s, size_t count) {
 bool all_elementwise(const dq_state *s) { return all_elementwise(s, s->queue.size()); }
 
 // Whether a queued op is unitary: every entry of U^H U - I within 1e-13 (the
-// C ABI takes any matrix, so this is checked, not assumed).  Selector-fused
-// ops and collapses are not.
+// C ABI takes any matrix, so this is checked, not assumed; a NaN or infinite
+// entry fails).  Selector-fused ops and collapses are not.
 bool queued_unitary(const HostOp &h) {
     if (h.kind == K_COLLAPSE || h.nsel > 0 || h.k < 1 || h.k > 5) return false;
     constexpr double tol = 1e-13;
@@ -929,7 +929,7 @@ bool queued_unitary(const HostOp &h) {
     if (h.kind == K_DIAG || h.kind == K_MONO) {
         if (h.mat.size() != size_t(2 * dim)) return false;
         for (int r = 0; r < dim; ++r)
-            if (std::fabs(h.mat[2 * r] * h.mat[2 * r] + h.mat[2 * r + 1] * h.mat[2 * r + 1] - 1.0) > tol) return false;
+            if (!(std::fabs(h.mat[2 * r] * h.mat[2 * r] + h.mat[2 * r + 1] * h.mat[2 * r + 1] - 1.0) <= tol)) return false;
         if (h.kind == K_DIAG) return true;
         if (h.perm.size() != size_t(dim)) return false;
         std::vector<char> seen(dim, 0);
@@ -959,7 +959,7 @@ bool queued_unitary(const HostOp &h) {
                 re += x[0] * y[0] + x[1] * y[1];
                 im += x[0] * y[1] - x[1] * y[0];
             }
-            if (std::fabs(re - (a == b ? 1.0 : 0.0)) > tol || std::fabs(im) > tol) return false;
+            if (!(std::fabs(re - (a == b ? 1.0 : 0.0)) <= tol && std::fabs(im) <= tol)) return false;  // NaN fails
         }
     return true;
 }

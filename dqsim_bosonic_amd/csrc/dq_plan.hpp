@@ -4,6 +4,7 @@
 // into LDS-tiled sweeps, and the CPU test-suite exercises it directly.
 #pragma once
 #include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -633,6 +634,8 @@ inline int op_variant(const std::vector<double> &m, int dim) {
 inline int phase_factor4(std::vector<double> &m) {
     const size_t nblk = m.size() / 32;
     if (nblk == 0 || m.size() % 32) return V_GEN;
+    for (const double x : m)
+        if (!std::isfinite(x)) return V_GEN;  // NaN / inf entries stay general
     for (const int form : {V_ROWPH, V_COLPH}) {
         std::vector<double> enc(m.size(), 0.0);
         bool ok = true;
@@ -659,7 +662,7 @@ inline int phase_factor4(std::vector<double> &m) {
                 for (int j = 0; j < 4; ++j) {
                     const double ur = u[2 * at(j)], ui = u[2 * at(j) + 1];
                     const double wr = ur * pr + ui * pi, wi = ui * pr - ur * pi;  // u * conj(phase)
-                    if (std::fabs(wi) > 4e-15 * am) {
+                    if (!(std::fabs(wi) <= 4e-15 * am)) {
                         ok = false;
                         break;
                     }

@@ -297,3 +297,14 @@ def test_phase_forms_only_with_pair_fusion():
         variants[fuse] = vs
     assert variants[2].count(4) >= len(variants[2]) // 2
     assert 4 not in variants[1] and 5 not in variants[1]
+
+
+def test_phase_forms_reject_non_finite_entries():
+    """A NaN or infinite entry keeps a 4x4 general (its encoding would not
+    reproduce it)."""
+    rq, _ = np.linalg.qr(np.random.default_rng(2).normal(size=(4, 4)))
+    for bad in (np.nan, np.inf):
+        u = rq.astype(np.complex128)
+        u[1, 2] = bad
+        v, _ = core._phase_factor4(_flat(u))
+        assert v == 0
