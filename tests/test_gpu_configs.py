@@ -15,7 +15,9 @@
 * The complex64 precision family: random_blocks (remote-link gates, measure,
   reset, conditional x) at 16 qubits with fusion level 3 (5-qubit blocks on
   FP32 MFMA) against complex128 at fusion level 1: records equal, amplitudes
-  within 2e-5 (FP32 unit roundoff 6e-8 over ~1000 gates).
+  within 2e-5 (FP32 unit roundoff 6e-8 over ~1000 gates).  And the precision
+  workload at its stated size, 32 qubits and depth 60, complex64 against
+  complex128 on the same GPU.
 """
 from __future__ import annotations
 
@@ -97,6 +99,39 @@ def test_random_blocks_30q_against_golden(fusion):
     assert err <= 1e-12
     del res, eng
     gc.collect()
+
+
+def test_precision_workload_32q_depth60():
+    """The precision workload at its stated size: 32 qubits, depth 60, circuit
+    and measurement seed 42, complex64 at fusion level 3 (5-qubit blocks on
+    FP32 MFMA; 32 GiB) against complex128 at fusion level 2 (64 GiB) on the
+    same GPU: measurement records equal, p1 within 1e-5, max |amplitude
+    difference| within 2e-5, fidelity deficit within 1e-5 and total
+    variation distance of the probabilities within 1e-4; the MFMA blocks ran."""
+    n = 32
+    text = W.random_blocks(n, 60, seed=42).model_dump_json()
+    try:
+        e32 = _core.Engine(n, 3, 0, 0, 1)
+        e64 = _core.Engine(n, 2, 0, 0, 0)
+    except MemoryError:
+        pytest.skip("needs about 100 GiB of device memory")
+    e32.load(text)
+    f0 = e32.fused_stats()
+    r32 = e32.step(42)
+    blocks = e32.fused_stats()[0] - f0[0]
+    e64.load(text)
+    r64 = e64.step(42)
+    try:
+        assert r32 == r64 and len(r32) == 12
+        assert np.max(np.abs(np.array(e32.p1_record()) - np.array(e64.p1_record()))) <= 1e-5
+        d = e32.compare(e64)
+        assert d["max_abs"] <= 2e-5
+        assert 1.0 - d["fidelity"] <= 1e-5
+        assert d["tvd"] <= 1e-4
+        assert blocks > 0
+    finally:
+        del e32, e64
+        gc.collect()
 
 
 @pytest.mark.parametrize("min_cost", [None, 0.0])
