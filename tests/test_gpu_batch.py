@@ -268,3 +268,26 @@ def test_autotuner_choice_and_results():
     assert st["persistent"] + st["engine"] == 3
     for e, (n, gates) in zip(got, insts):
         assert abs(e - expect_z(oracle_state(n, gates), zz_chain(n))) <= 1e-12 * max(1.0, abs(e))
+
+
+def test_batch_workload_4096_at_stated_size():
+    """The batching workload at its stated size: the 4096 HEA circuits (n ~
+    U{10..24}, depth ~ U{8..64}, seed 42) through the autotuned paths (one
+    captured graph for the LDS-resident and persistent instances, engine
+    workers for the rest), complex128 and complex64.  Every complex64 <ZZ>
+    within 1e-5 of the complex128 one; complex128 against the oracle's
+    statevector to 1e-12 for the shallowest instance of each width 10..16."""
+    from dqsim_bosonic_amd import _core
+
+    ns, ds, angles = W.hea_batch_arrays(4096, 42)
+    flat = np.concatenate([a.ravel() for a in angles])
+    e128, st = _core.simulate_hea_batch(ns, ds, flat, dtype="c128", workers=8)
+    e64, _ = _core.simulate_hea_batch(ns, ds, flat, dtype="c64", workers=8)
+    assert int(st["lds_instances"]) + int(st["persistent_instances"]) + int(st["engine_instances"]) == 4096
+    assert int(st["engine_instances"]) > 0 and int(st["persistent_instances"]) > 0
+    assert np.max(np.abs(np.asarray(e64) - np.asarray(e128))) <= 1e-5
+    for n in range(10, 17):
+        idx = np.flatnonzero(ns == n)
+        i = int(idx[np.argmin(ds[idx])])
+        want = expect_z(oracle_state(n, hea_gates(n, int(ds[i]), angles[i])), zz_chain(n))
+        assert abs(e128[i] - want) <= 1e-12 * max(1.0, abs(want)), (n, i)
