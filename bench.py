@@ -20,7 +20,10 @@ for a draw within rounding of p1; near_ties counts draws within 1e-12.
 
 roofline: the dominant kernel is the LDS-tiled sweep (k_tile_reg); its
 launches are bracketed by HIP events on the engine's stream.  Algorithmic
-bytes per launch = one read + one write of every amplitude = 2 * 16 * 2^n.
+bytes per launch = one read + one write of every amplitude the launch visits,
+2 * 16 * (amplitudes visited) / launches: 2^n per sweep, except the first
+sweeps after each reset, which visit only the tiles where the qubits no op has
+touched yet are 0 (the other tiles hold zeros and stay zero).
 
 cpu_baseline: the oracle's OpenMP restatement of the reference's Rayon
 kernels (src/engine.rs) on the host cores, on a bounded sample: the circuit's
@@ -319,6 +322,7 @@ def precision_leg(n: int, depth: int, seed: int, steps: int, modes: list[str] | 
         rec32 = e32.step(seed)
     dt32 = (time.perf_counter() - t0) / steps
     ms32, cnt32 = e32.timing()
+    swept32 = e32.timed_amplitudes()
     f1 = e32.fused_stats()
     s1 = e32.stats()[0]
     e32.set_timing(False)
@@ -332,7 +336,7 @@ def precision_leg(n: int, depth: int, seed: int, steps: int, modes: list[str] | 
     cmp = e32.compare(e64)
     avg32 = ms32 / max(cnt32, 1)
     blocks = (f1[0] - f0[0]) / steps
-    alg_bytes = 2.0 * 8.0 * float(1 << n)
+    alg_bytes = 2.0 * 8.0 * swept32 / max(cnt32, 1)
     # complex 32x32 GEMM per 32-amplitude group: 32*32 complex MACs = 8192 flops
     gemm_flops_per_sweep = blocks / max((s1 - s0) / steps, 1) * 8192.0 * float(1 << (n - 5))
     out.update({
@@ -499,6 +503,7 @@ def main() -> int:
         log(f"[bench] step {k} done")
     t1 = time.perf_counter()
     kernel_ms, timed = eng.timing()
+    swept = eng.timed_amplitudes()
     sweeps1, launches1, _ = eng.stats()
     eng.set_timing(False)
     assert all(r == records[0] for r in records), "measurement record changed between steps"
@@ -529,7 +534,10 @@ def main() -> int:
     value = applied * args.steps / elapsed
     sweeps_per_step = (sweeps1 - sweeps0) / args.steps
     avg_launch_ms = kernel_ms / max(timed, 1)
-    alg_bytes = 2.0 * 16.0 * float(1 << n)
+    # one read and one write of each amplitude a sweep visits: 2^n per sweep,
+    # less in the first sweeps after the reset, which visit only the tiles
+    # where qubits no op has touched yet are 0
+    alg_bytes = 2.0 * 16.0 * swept / max(timed, 1)
     achieved = alg_bytes / (avg_launch_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(n),
@@ -538,6 +546,8 @@ def main() -> int:
                 "kernel": f"k_tile_reg{'8' if args.reg_bits == 3 else ''}<double,{args.tile_bits or 12}>",
                 "avg_launch_ms": round(avg_launch_ms, 4),
                 "launches_timed": int(timed), "sweeps_per_step": sweeps_per_step,
+                "alg_bytes_per_launch": round(alg_bytes), "amplitudes_per_launch_frac_of_state":
+                    round(swept / max(timed, 1) / float(1 << n), 4),
                 "kernel_time_frac_of_step": round(kernel_ms / 1e3 / elapsed, 4)}
 
     adj = None

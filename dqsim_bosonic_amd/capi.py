@@ -24,7 +24,8 @@ EXPORTS = (
     "dq_marginal_probs_noalloc", "dq_inner",
     "dq_copy_to_host", "dq_synchronize", "dq_state_copy", "dq_apply_pauli_sum", "dq_pauli_inner",
     "dq_compare_states", "dq_state_tensor", "dq_adjoint_sweep",
-    "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_stats",
+    "dq_set_options", "dq_set_timing", "dq_get_timing", "dq_get_timed_amplitudes", "dq_set_zero_tiles",
+    "dq_get_stats",
     "dq_get_fused_stats", "dq_set_fuse_min_cost", "dq_set_pair_fusion", "dq_set_reg_bits", "dq_top_k",
     "dq_batch_create", "dq_batch_destroy", "dq_batch_add_instance", "dq_batch_apply",
     "dq_batch_set_observable", "dq_batch_set_workers", "dq_batch_run", "dq_batch_stats",
@@ -101,6 +102,8 @@ def lib() -> ctypes.CDLL:
         L.dq_pauli_inner.argtypes = [vp, vp, ctypes.c_int, u64p, u64p, _dp, _dp, _dp]
         L.dq_set_timing.argtypes = [vp, ctypes.c_int]
         L.dq_get_timing.argtypes = [vp, _dp, u64p]
+        L.dq_get_timed_amplitudes.argtypes = [vp, u64p]
+        L.dq_set_zero_tiles.argtypes = [vp, ctypes.c_int]
         L.dq_mps_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
         L.dq_mps_destroy.argtypes = [vp]
         L.dq_mps_destroy.restype = None
@@ -260,6 +263,20 @@ class State:
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().dq_get_fused_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_zero_tiles(self, on: bool) -> None:
+        """Whether register sweeps skip the tiles where a qubit still 0
+        everywhere (untouched since the last reset) is 1."""
+        _check(lib().dq_set_zero_tiles(self.h, int(bool(on))))
+
+    def set_timing(self, on: bool) -> None:
+        _check(lib().dq_set_timing(self.h, int(bool(on))))
+
+    def timed_amplitudes(self) -> int:
+        """Amplitudes the sweep kernels read and wrote since set_timing."""
+        a = ctypes.c_uint64()
+        _check(lib().dq_get_timed_amplitudes(self.h, ctypes.byref(a)))
+        return a.value
 
     def tensor(self, a: "State", a_pos, b: "State", b_pos) -> None:
         """self = a (x) b, local qubit k of a at position a_pos[k], of b at b_pos[k]."""

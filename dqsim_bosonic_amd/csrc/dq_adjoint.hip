@@ -1457,6 +1457,9 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
     if (n_ops == 0) return DQ_OK;
     if (int rc = dq_synchronize(psi)) return rc;
     if (int rc = dq_synchronize(lam)) return rc;
+    // the sweep rewrites both states in place
+    psi->zero_q = 0;
+    lam->zero_q = 0;
     const int b = std::min(n, 11);
     const uint64_t low = (uint64_t(1) << kLowBits) - 1;
     // passes: ops in reverse order; an op joins the pass if its qubits fit
@@ -1730,6 +1733,7 @@ int dq_adjoint_sweep(dq_state *psi, dq_state *lam, int n_ops, const int *k, cons
             slot_base += ns;
             psi->launches++;
             psi->sweeps++;
+            psi->swept_amps += uint64_t(1) << n;
         }
         double *hg = reinterpret_cast<double *>(psi->h_adj + b_tables);
         ADJ_TRY(hipMemcpyAsync(hg, d_grads, slots_total * sizeof(double), hipMemcpyDeviceToHost, stream));

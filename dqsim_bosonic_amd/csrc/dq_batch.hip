@@ -605,6 +605,7 @@ struct PersistPlan {
     std::vector<TileOp> ops;
     std::vector<double> mats;
     std::vector<uint32_t> lays;
+    double swept_amps = 0.0;  // amplitudes the passes read and write (tiles visited x tile size)
     bool ok = true;
 };
 
@@ -630,6 +631,9 @@ PersistPlan plan_persist(const BInstance &in, unsigned elem) {
         classify(op, g.u.data());
         all.push_back(std::move(op));
     }
+    // the slot starts as |0...0>: qubits no pass has touched yet are 0
+    // everywhere, and each pass visits only the tiles where they are 0
+    uint64_t zero_q = ~uint64_t(0) >> (64 - in.n);
     for (size_t c0 = 0; c0 < all.size(); c0 += 4096) {
         std::vector<HostOp> queue(all.begin() + long(c0), all.begin() + long(std::min(all.size(), c0 + 4096)));
         for (const Pass &p : schedule(&cfg, queue, true)) {
@@ -638,7 +642,9 @@ PersistPlan plan_persist(const BInstance &in, unsigned elem) {
                 return pp;
             }
             RegPlan rp;
-            plan_pass(&cfg, queue, p, pp.mats, rp);
+            plan_pass(&cfg, queue, p, pp.mats, rp, zero_q);
+            zero_q &= ~pass_support(queue, p);
+            pp.swept_amps += std::ldexp(1.0, rp.b + rp.pd.n_outer);
             if (rp.b != kPersistTileBits) {
                 pp.ok = false;
                 return pp;
@@ -723,7 +729,7 @@ void build_persist(const dq_batch *b, const std::vector<int> &ids, std::vector<P
         zm.insert(zm.end(), in.zmask.begin(), in.zmask.end());
         zc.insert(zc.end(), in.zcoef.begin(), in.zcoef.end());
         tb.max_n = std::max(tb.max_n, in.n);
-        tb.bytes += (2.0 * double(pp.passes.size()) + 2.0) * double(sizeof(cplx<T>)) * std::ldexp(1.0, in.n);
+        tb.bytes += (2.0 * pp.swept_amps + 2.0 * std::ldexp(1.0, in.n)) * double(sizeof(cplx<T>));
     }
     tb.n_items = int(items.size());
     // the layout tables follow the matrices; a pass finds its own at lay_off bytes from the matrices
@@ -1024,8 +1030,7 @@ int run_engine(dq_batch *b, const std::vector<int> &ids, double *out,
                 auto tn = tuned.find(in.n);
                 if (rc == DQ_OK && tn != tuned.end()) rc = engine_config(s, b->dtype, tn->second);
             }
-            uint64_t sw0 = 0, sw1 = 0;
-            if (rc == DQ_OK) rc = dq_get_stats(s, &sw0, nullptr, nullptr);
+            const uint64_t sw0 = s->swept_amps;
             if (rc == DQ_OK) rc = dq_state_reset(s);
             for (size_t gi = 0; rc == DQ_OK && gi < in.gates.size(); ++gi) {
                 const BGate &g = in.gates[gi];
@@ -1040,17 +1045,18 @@ int run_engine(dq_batch *b, const std::vector<int> &ids, double *out,
                 for (int t = 0; t < nt; ++t) c[2 * size_t(t)] = in.zcoef[size_t(t)];
                 rc = nt ? dq_pauli_inner(s, s, nt, xm.data(), in.zmask.data(), c.data(), &re, &im) : DQ_OK;
             }
-            if (rc == DQ_OK) rc = dq_get_stats(s, &sw1, nullptr, nullptr);
             if (rc != DQ_OK) {
                 record(rc, dq_last_error());
                 break;
             }
             out[ids[i]] = re;
             {
-                // one read and one write per sweep, the reset's write and the expectation's read
-                const double amps_bytes = std::ldexp(double(b->dtype == DQ_C128 ? 16 : 8), in.n);
+                // one read and one write of each amplitude a sweep visits, the
+                // reset's write and the expectation's read
+                const double elem = b->dtype == DQ_C128 ? 16.0 : 8.0;
+                const double swept = double(s->swept_amps - sw0);
                 std::lock_guard<std::mutex> lk(mu);
-                b->bytes[2] += (2.0 * double(sw1 - sw0) + 2.0) * amps_bytes;
+                b->bytes[2] += (2.0 * swept + 2.0 * std::ldexp(1.0, in.n)) * elem;
             }
         }
         for (auto &kv : states)

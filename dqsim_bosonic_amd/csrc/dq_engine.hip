@@ -737,6 +737,7 @@ int launch_tile(dq_state *s, const PassDesc &pd, const DevOp *ops, const void *m
     }
 #undef DQ_TILE_CASE
     HIP_TRY(hipGetLastError());
+    s->swept_amps += amp_count(s);
     s->launches++;
     s->sweeps++;
     return DQ_OK;
@@ -750,6 +751,7 @@ int launch_direct(dq_state *s, const DevOp *ops, int index, const void *mats, co
                        static_cast<cplx<T> *>(s->amps), s->n, ops, index,
                        static_cast<const cplx<T> *>(mats), perms);
     HIP_TRY(hipGetLastError());
+    s->swept_amps += amp_count(s);
     s->launches++;
     s->sweeps++;
     return DQ_OK;
@@ -778,6 +780,7 @@ int launch_reg(dq_state *s, const RegPlan &rp, const TileOp *ops, const void *ma
     s->last_grid = int(grid);
     if (e == hipErrorInvalidValue) return fail(DQ_ERR_ARG, "unsupported tile size " + std::to_string(rp.b));
     HIP_TRY(e);
+    s->swept_amps += ntiles << rp.b;
     s->launches++;
     s->sweeps++;
     return DQ_OK;
@@ -791,20 +794,21 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     std::vector<RegPlan> plans(passes.size());
     std::vector<double> mats;
     size_t total_ops = 0;
+    // qubits still 0 everywhere before each pass; a pass's sweep skips the
+    // tiles where any outer one of them is 1 (they hold zeros and stay zero)
+    uint64_t zero_q = s->zero_tiles ? s->zero_q : 0;
     for (size_t i = 0; i < passes.size(); ++i) {
-        plan_pass(&cfg, s->queue, passes[i], mats, plans[i]);
+        plan_pass(&cfg, s->queue, passes[i], mats, plans[i], zero_q);
+        zero_q &= ~pass_support(s->queue, passes[i]);
         s->fused_blocks += uint64_t(plans[i].fused_blocks);
         s->fused_gates += uint64_t(plans[i].fused_gates);
         plans[i].pd.op_begin = int(total_ops);
         plans[i].pd.op_count = int(plans[i].ops.size());
         plans[i].pd.red_bit = -1;
         if (red_qubit >= 0 && i + 1 == passes.size()) {
-            int ob = 0;
-            for (int q = 0; q < s->n; ++q) {
-                if ((passes[i].S >> q) & 1) continue;
-                if (q == red_qubit) plans[i].pd.red_bit = kOuterBit + ob;
-                ++ob;
-            }
+            // an outer bit past n_outer reads 0 in every visited tile: p1 = +0.0
+            for (int ob = 0; ob < s->n - plans[i].b; ++ob)
+                if (plans[i].pd.oq[ob] == red_qubit) plans[i].pd.red_bit = kOuterBit + ob;
             for (int tb = 0; tb < plans[i].b; ++tb)
                 if (plans[i].pd.tq[tb] == red_qubit) plans[i].pd.red_bit = tb;
         }
@@ -837,8 +841,10 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     HIP_TRY(hipEventRecord(s->upload_done, s->stream));
     const TileOp *d_ops = reinterpret_cast<const TileOp *>(s->d_upload);
     const void *d_mats = static_cast<unsigned char *>(s->d_upload) + ops_b;
+    s->zero_q = 0;  // unknown should a launch fail part way
     for (const RegPlan &rp : plans)
         if (int rc = launch_reg<T>(s, rp, d_ops, d_mats)) return rc;
+    s->zero_q = zero_q;
     s->queue.clear();
     return DQ_OK;
 }
@@ -854,6 +860,7 @@ int flush_t(dq_state *s, int red_qubit = -1) {
     if (s->fuse && s->reg_tiles && effective_tile_bits(&cfg) >= 9) {
         return flush_reg<T>(s, schedule(&cfg, s->queue, true), red_qubit);
     }
+    for (const HostOp &h : s->queue) s->zero_q &= ~op_support(h);
     if (s->fuse && n >= 1) {
         std::vector<Pass> passes = schedule(&cfg, s->queue);
         for (const Pass &p : passes) {
@@ -1085,12 +1092,14 @@ int dq_state_reset(dq_state *s) {
                            static_cast<cplx<float> *>(s->amps), count);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
+    s->zero_q = s->n ? ~uint64_t(0) >> (64 - s->n) : 0;
     return DQ_OK;
 }
 
 int dq_state_set(dq_state *s, const double *src) {
     if (!s || !src) return fail(DQ_ERR_ARG, "null argument");
     s->queue.clear();
+    s->zero_q = 0;
     const uint64_t count = amp_count(s);
     HIP_TRY(hipStreamSynchronize(s->stream));
     if (s->dtype == DQ_C128) {
@@ -1529,6 +1538,7 @@ int dq_state_copy(dq_state *dst, dq_state *src) {
     if (dst == src) return flush(dst);
     if (int rc = join(dst, src)) return rc;
     dst->queue.clear();
+    dst->zero_q = src->zero_q;
     const uint64_t count = amp_count(dst);
     if (dst->dtype == DQ_C128)
         hipLaunchKernelGGL(k_copy_state<double>, dim3(grid_for(count)), dim3(kThreads), 0, dst->stream,
@@ -1562,6 +1572,12 @@ int dq_state_tensor(dq_state *dst, dq_state *a, const int *a_pos, dq_state *b, c
     if (int rc = join(dst, a)) return rc;
     if (int rc = join(dst, b)) return rc;
     dst->queue.clear();
+    // a product amplitude is zero wherever either factor's is
+    dst->zero_q = 0;
+    for (int k = 0; k < a->n; ++k)
+        if ((a->zero_q >> k) & 1) dst->zero_q |= uint64_t(1) << tp.apos[k];
+    for (int k = 0; k < b->n; ++k)
+        if ((b->zero_q >> k) & 1) dst->zero_q |= uint64_t(1) << tp.bpos[k];
     const uint64_t count = amp_count(dst);
     if (dst->dtype == DQ_C128)
         hipLaunchKernelGGL(k_tensor<double>, dim3(grid_for(count)), dim3(kThreads), 0, dst->stream,
@@ -1587,6 +1603,7 @@ int dq_apply_pauli_sum(dq_state *dst, dq_state *src, int n_terms, const uint64_t
     if (int rc = pauli_terms(src, n_terms, xmask, zmask, coeff, terms)) return rc;
     if (int rc = join(dst, src)) return rc;
     dst->queue.clear();
+    dst->zero_q = 0;
     PauliTerm *d_terms;
     if (int rc = upload_terms(dst, terms, &d_terms)) return rc;
     const uint64_t count = amp_count(dst);
@@ -1878,6 +1895,7 @@ int dq_set_timing(dq_state *s, int on) {
     s->timing = on ? 1 : 0;
     s->timed_ms = 0.0;
     s->timed_launches = 0;
+    s->swept_amps0 = s->swept_amps;
     return DQ_OK;
 }
 
@@ -1887,6 +1905,20 @@ int dq_get_timing(dq_state *s, double *sweep_ms, uint64_t *sweep_launches) {
     if (int rc = collect_timing(s)) return rc;
     if (sweep_ms) *sweep_ms = s->timed_ms;
     if (sweep_launches) *sweep_launches = s->timed_launches;
+    return DQ_OK;
+}
+
+int dq_set_zero_tiles(dq_state *s, int on) {
+    if (!s) return fail(DQ_ERR_ARG, "null state");
+    if (int rc = flush(s)) return rc;
+    s->zero_tiles = on ? 1 : 0;
+    return DQ_OK;
+}
+
+int dq_get_timed_amplitudes(dq_state *s, uint64_t *amps) {
+    if (!s || !amps) return fail(DQ_ERR_ARG, "null argument");
+    if (int rc = flush(s)) return rc;
+    *amps = s->swept_amps - s->swept_amps0;
     return DQ_OK;
 }
 

@@ -475,6 +475,7 @@ int dq_mps_to_state(dq_mps *M, dq_state *dst) {
     if (dst->n != M->n) return mfail(DQ_ERR_ARG, "state and MPS differ in qubit count");
     if (dst->device != M->device) return mfail(DQ_ERR_ARG, "state and MPS live on different devices");
     dst->queue.clear();
+    dst->zero_q = 0;
     MPS_TRY(hipStreamSynchronize(dst->stream));
     if (int rc = grow(M)) return rc;
     uint64_t nb = 0;

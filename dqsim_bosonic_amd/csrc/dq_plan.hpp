@@ -820,9 +820,33 @@ inline void append_layout_table(std::vector<uint32_t> &out, int rb, int b, unsig
     }
 }
 
-// Plans the layout of every op of one pass and encodes the ops.
+// Qubits on which a queued op can turn an amplitude that is zero at 0 into a
+// nonzero one: every qubit it acts on, except those on which it acts
+// diagonally (a diagonal op or a collapse scales each amplitude by a factor;
+// a controlled op acts on its target only where its controls match).
+inline uint64_t op_support(const HostOp &h) {
+    if (h.kind == K_DIAG || h.kind == K_COLLAPSE) return 0;
+    if (h.kind == K_CTRL1) return uint64_t(1) << h.q[h.k - 1];
+    return h.qmask();
+}
+
+// Qubits a register pass can leave nonzero: its tile qubits and the support
+// of each of its ops.
+inline uint64_t pass_support(const std::vector<HostOp> &queue, const Pass &p) {
+    uint64_t m = p.S;
+    for (int i : p.ops) m |= op_support(queue[size_t(i)]);
+    return m;
+}
+
+// Plans the layout of every op of one pass and encodes the ops.  zero_q:
+// qubits that are 0 in every amplitude before the pass.  Outer qubits among
+// them take the highest outer-index bits and are left out of pd.n_outer, so a
+// launch over 2^n_outer tiles visits exactly the tiles where they are 0 (the
+// others hold zeros, which the pass would leave zero), and ops that read such
+// a qubit's outer bit (controls, selectors, diagonals, collapses) read its
+// value there, 0.
 inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, const Pass &p, std::vector<double> &mats,
-               RegPlan &out) {
+               RegPlan &out, uint64_t zero_q = 0) {
     const int n = s->n;
     TilePass &pd = out.pd;
     std::memset(&pd, 0, sizeof pd);
@@ -832,17 +856,22 @@ inline void plan_pass(const PlanConfig *s, const std::vector<HostOp> &queue, con
         if ((p.S >> q) & 1) {
             pd.tq[tb] = q;
             pos[q] = tb++;
-        } else {
+        } else if (!((zero_q >> q) & 1)) {
             pd.oq[ob] = q;
             pos[q] = kOuterBit + ob++;
         }
     }
+    pd.n_outer = ob;
+    for (int q = 0; q < n; ++q)
+        if (!((p.S >> q) & 1) && ((zero_q >> q) & 1)) {
+            pd.oq[ob] = q;
+            pos[q] = kOuterBit + ob++;
+        }
     const int b = tb;
     out.b = b;
     const int R = s->reg_bits;  // register bits per layout
     out.reg_bits = R;
     auto inner = [&](int bitpos) { return bitpos < kOuterBit; };
-    pd.n_outer = ob;
     // the low qubits stay thread bits at load/store (coalesced runs)
     const unsigned lowmask = (1u << std::max(0, std::min(s->low_bits, b - R))) - 1;
     const unsigned dflt = ((1u << b) - 1) & ~((1u << (b - R)) - 1);

@@ -21,6 +21,11 @@ struct dq_state {
     int reg_tiles = 1;  // register-resident tile kernel for tiles of >= 9 qubits
     int last_grid = 0;  // grid of the last register-tile launch (partials it wrote)
     int low_bits = 4;
+    // qubits that are 0 in every amplitude of the state in memory (the queue
+    // not yet applied): all of them after a reset, none after an upload;
+    // register sweeps visit only the tiles where these qubits are 0
+    uint64_t zero_q = 0;
+    int zero_tiles = 1;  // use zero_q to skip tiles (dq_set_zero_tiles)
     std::vector<dq::HostOp> queue;
     // upload buffers
     void *d_upload = nullptr;
@@ -50,4 +55,7 @@ struct dq_state {
     std::vector<hipEvent_t> ev_pool;
     double timed_ms = 0.0;
     uint64_t timed_launches = 0;
+    // amplitudes the sweep kernels have read and written (each once) so far,
+    // and the count when timing was last switched
+    uint64_t swept_amps = 0, swept_amps0 = 0;
 };

@@ -275,6 +275,8 @@ PYBIND11_MODULE(_core, m) {
         .def("step", &Engine::step, py::arg("seed"))
         .def("set_timing", &Engine::set_timing)
         .def("timing", &Engine::timing)
+        .def("timed_amplitudes", &Engine::timed_amplitudes)
+        .def("set_zero_tiles", &Engine::set_zero_tiles)
         .def("stats", &Engine::stats)
         .def("fused_stats", &Engine::fused_stats)
         .def("set_fuse_min_cost", &Engine::set_fuse_min_cost)
@@ -474,6 +476,31 @@ PYBIND11_MODULE(_core, m) {
             d["relayouts"] = rp.relayouts;
             d["ops"] = tops;
             out.append(d);
+        }
+        return out;
+    }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,
+       py::arg("reg_bits") = 4);
+    // The register passes of an op list applied to |0...0>, as a flush plans
+    // them: per pass (tile mask, op indices, qubits 0 everywhere before it,
+    // n_outer, outer qubits in outer-index order).
+    m.def("_zero_passes", [](int n, py::list ops, int fuse, int tile_bits, int low_bits, int reg_bits) {
+        std::vector<dq::HostOp> queue = host_ops(ops);
+        dq::PlanConfig cfg;
+        cfg.n = n;
+        cfg.fuse = fuse;
+        cfg.tile_bits = tile_bits;
+        cfg.low_bits = low_bits;
+        cfg.reg_bits = reg_bits;
+        if (!(fuse && dq::effective_tile_bits(&cfg) >= 9)) throw std::invalid_argument("not a register-tile plan");
+        uint64_t zero_q = n ? ~uint64_t(0) >> (64 - n) : 0;
+        py::list out;
+        std::vector<double> mats;
+        for (const dq::Pass &p : dq::schedule(&cfg, queue, true)) {
+            dq::RegPlan rp;
+            dq::plan_pass(&cfg, queue, p, mats, rp, zero_q);
+            std::vector<int> oq(rp.pd.oq, rp.pd.oq + (n - rp.b));
+            out.append(py::make_tuple(p.S, p.ops, zero_q, rp.pd.n_outer, oq));
+            zero_q &= ~dq::pass_support(queue, p);
         }
         return out;
     }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,

@@ -178,6 +178,12 @@ class Engine {
         check(dq_get_timing(st_->s, &ms, &cnt), "timing");
         return py::make_tuple(ms, cnt);
     }
+    uint64_t timed_amplitudes() {
+        uint64_t a = 0;
+        check(dq_get_timed_amplitudes(st_->s, &a), "timed amplitudes");
+        return a;
+    }
+    void set_zero_tiles(bool on) { check(dq_set_zero_tiles(st_->s, on ? 1 : 0), "zero tiles"); }
     py::tuple stats() {
         uint64_t a = 0, b = 0, c = 0;
         check(dq_get_stats(st_->s, &a, &b, &c), "stats");

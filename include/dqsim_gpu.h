@@ -178,6 +178,19 @@ int dq_set_options(dq_state *s, int fuse, int tile_bits, int low_bits);
 int dq_set_timing(dq_state *s, int on);
 int dq_get_timing(dq_state *s, double *sweep_ms, uint64_t *sweep_launches);
 
+/* Amplitudes the state-sweep kernels read and wrote (each once per sweep)
+ * since dq_set_timing (no reference counterpart; measurement only).  A sweep
+ * visits only the tiles where qubits that are still 0 in every amplitude
+ * (none touched since the last dq_state_reset) are 0, so this can be less
+ * than sweeps x 2^n. */
+int dq_get_timed_amplitudes(dq_state *s, uint64_t *amps);
+
+/* Whether register sweeps skip the tiles where a qubit still 0 in every
+ * amplitude is 1 (default on; no reference counterpart).  Those tiles hold
+ * zeros that a sweep would leave zero, so either setting gives the same
+ * state; off is for timing comparisons. */
+int dq_set_zero_tiles(dq_state *s, int on);
+
 /* Number of HBM sweeps and kernel launches issued so far on this handle. */
 int dq_get_stats(const dq_state *s, uint64_t *sweeps, uint64_t *launches,
                  uint64_t *gates);

@@ -259,3 +259,71 @@ def test_layout_order_groups_register_needs():
         return count
 
     assert relayouts(order) < relayouts(range(len(ops)))
+
+
+def spreading_ops(n, count, rng):
+    """Ops whose qubits spread upward from the low ones (op i draws from the
+    lowest 3 + i // 8 qubits), plus diagonals and controlled 2x2s whose
+    controls may sit on qubits no op has touched yet (control value 0 or 1)."""
+    ops = []
+    for i in range(count):
+        hi = min(n, 3 + i // 8)
+        r = rng.random()
+        if r < 0.15:
+            qs = [int(x) for x in rng.choice(n, size=2, replace=False)]
+            ops.append((qs, orc.gate_matrix("cp", rng.uniform(-4, 4, 1)), []))
+        elif r < 0.3:
+            c = int(rng.integers(n))
+            t = int(rng.choice([q for q in range(hi) if q != c]))
+            ops.append(([c, t], orc.gate_matrix("u3", rng.uniform(-4, 4, 3)), [int(rng.integers(2))]))
+        else:
+            ops.extend(random_ops(hi, 1, rng))
+    return ops
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("n,tile_bits,low_bits", [(14, 9, 4), (16, 10, 3)])
+def test_zero_qubits_before_each_pass(seed, n, tile_bits, low_bits):
+    """Applied to |0...0>, the qubits a flush takes as 0 before a pass are 0
+    in every amplitude there (exact zeros in the oracle's statevector), the
+    outer index puts them after the others (ascending within each group) and
+    n_outer counts the others; the first pass visits a single tile."""
+    rng = np.random.default_rng(300 + seed)
+    ops = spreading_ops(n, 120, rng)
+    plan = core._zero_passes(n, [(qs, np.asarray(u).ravel(), cv) for qs, u, cv in ops], 2, tile_bits, low_bits)
+    assert plan[0][3] == 0
+    idx = np.arange(1 << n)
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[0] = 1.0
+    skipped = 0
+    prev = (1 << n) - 1
+    for S, pass_ops, zero_q, n_outer, oq in plan:
+        assert zero_q & ~prev == 0  # only shrinks
+        prev = zero_q
+        outer = [q for q in range(n) if not (S >> q) & 1]
+        live = [q for q in outer if not (zero_q >> q) & 1]
+        assert oq == live + [q for q in outer if (zero_q >> q) & 1]
+        assert n_outer == len(live)
+        assert np.all(psi[(idx & zero_q) != 0] == 0)
+        skipped += len(outer) - n_outer
+        for i in pass_ops:
+            psi = apply_op(psi, n, *ops[i])
+    assert skipped > 0
+    want = np.zeros(1 << n, dtype=complex)
+    want[0] = 1.0
+    for op in ops:
+        want = apply_op(want, n, *op)
+    assert np.max(np.abs(psi - want)) < 1e-12
+
+
+def test_zero_qubit_invariant_catches_a_wrong_support():
+    """The invariant above fails when a pass's ops are taken as leaving a
+    qubit they act on at 0: a hadamard on qubit 0 makes amplitudes with
+    bit 0 set nonzero."""
+    n = 4
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[0] = 1.0
+    psi = apply_op(psi, n, [0], orc.gate_matrix("h", []), [])
+    idx = np.arange(1 << n)
+    wrong_zero_q = 0b0001
+    assert not np.all(psi[(idx & wrong_zero_q) != 0] == 0)
