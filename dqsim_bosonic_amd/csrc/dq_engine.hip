@@ -1083,6 +1083,7 @@ int dq_state_num_qubits(const dq_state *s) { return s ? s->n : -1; }
 int dq_state_reset(dq_state *s) {
     if (!s) return fail(DQ_ERR_ARG, "null state");
     s->queue.clear();
+    s->zero_q = 0;  // known again once the fill has completed
     const uint64_t count = amp_count(s);
     if (s->dtype == DQ_C128)
         hipLaunchKernelGGL(k_fill_basis<double>, dim3(grid_for(count)), dim3(kThreads), 0, s->stream,

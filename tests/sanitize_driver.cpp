@@ -155,6 +155,8 @@ void check_plan(const dq::PlanConfig &cfg, const std::vector<dq::HostOp> &queue)
     const std::vector<dq::Pass> passes = dq::schedule(&cfg, queue, anywhere);
     std::vector<int> seen(queue.size(), 0);
     std::vector<double> mats;
+    // qubits 0 everywhere before each pass, as a flush from |0...0> tracks them
+    uint64_t zero_q = cfg.n ? ~uint64_t(0) >> (64 - cfg.n) : 0;
     for (const dq::Pass &p : passes) {
         for (int i : p.ops) {
             EXPECT(i >= 0 && size_t(i) < queue.size());
@@ -163,7 +165,23 @@ void check_plan(const dq::PlanConfig &cfg, const std::vector<dq::HostOp> &queue)
         if (!p.S || !anywhere) continue;
         EXPECT(__builtin_popcountll(p.S) == dq::effective_tile_bits(&cfg));
         dq::RegPlan rp;
-        dq::plan_pass(&cfg, queue, p, mats, rp);
+        dq::plan_pass(&cfg, queue, p, mats, rp, zero_q);
+        {
+            // the outer index lists every outer qubit once, the ones not known
+            // to be 0 first; n_outer counts those
+            uint64_t seen_o = 0;
+            int live = 0;
+            for (int j = 0; j < cfg.n - rp.b; ++j) {
+                const int q = rp.pd.oq[j];
+                EXPECT(q >= 0 && q < cfg.n && !((p.S >> q) & 1) && !((seen_o >> q) & 1));
+                if (q < 0 || q >= cfg.n) break;
+                seen_o |= uint64_t(1) << q;
+                EXPECT(((zero_q >> q) & 1) == (j >= rp.pd.n_outer));
+                live += !((zero_q >> q) & 1);
+            }
+            EXPECT(live == rp.pd.n_outer);
+        }
+        zero_q &= ~dq::pass_support(queue, p);
         const int R = rp.reg_bits;
         for (const dq::TileOp &t : rp.ops) {
             if (t.kind == dq::T_DENSE1 || t.kind == dq::T_CTRL1) EXPECT(t.p >= 0 && t.p < R);
