@@ -1186,6 +1186,17 @@ int dq_collapse(dq_state *s, int qubit, int outcome, double norm) {
 int dq_prob_one(dq_state *s, int qubit, double *p1) {
     if (!s || !p1) return fail(DQ_ERR_ARG, "null argument");
     if (int rc = check_qubit(s, qubit)) return rc;
+    // a qubit 0 in every amplitude in memory, on which no queued op acts
+    // other than diagonally, is still 0 everywhere: the sum over its 1-half
+    // is a sum of exact zeros, +0.0, with nothing flushed or read
+    if ((s->zero_q >> qubit) & 1) {
+        bool touched = false;
+        for (const HostOp &h : s->queue) touched = touched || ((op_support(h) >> qubit) & 1);
+        if (!touched) {
+            *p1 = 0.0;
+            return DQ_OK;
+        }
+    }
     // ops that decide p1 (the queue up to trailing unitaries on other qubits,
     // such as the x of a reset of another qubit); the rest stay queued
     const size_t live = marginal_prefix(s, qubit);

@@ -9,16 +9,19 @@ Programs whose qubits spread upward from the low ones, with diagonals and
 controlled gates whose controls sit on untouched qubits (control values 0
 and 1), and measurements/collapses of touched and untouched qubits:
 * the state equals the one of the same program with skipping off
-  (np.array_equal: the same ops on the same tiles; p1 within 1e-13, as a
-  sweep over fewer tiles sums its per-workgroup partials in other groups)
-  and the oracle's
+  (np.array_equal at fusion level 1: the same ops on the same tiles; within
+  1e-12 at level 2, where a measurement that needs no flush with skipping on
+  may end a flush with it off, and a flush's ops are reordered; p1 within
+  1e-13, as a sweep over fewer tiles sums its per-workgroup partials in other
+  groups) and the oracle's
   (src/engine.rs restated; bit-exact at fusion level 1, within 1e-12 at
   level 2, which reorders and contracts, within 1e-5 in complex64);
 * p1 of an untouched qubit is exactly 0.0, as the reference's sum of zeros;
   other p1 within 1e-13 relative (sums of up to 2^20 terms in another order);
 * with skipping on, the sweeps of a state larger than one 12-qubit tile
   read and write fewer than sweeps x 2^n amplitudes; with it off, exactly
-  that many.
+  that many;
+* p1 of such a qubit is +0.0 without a launch or a flush.
 """
 from __future__ import annotations
 
@@ -138,19 +141,24 @@ def test_zero_tiles_equal_full_sweeps_and_oracle(n, fuse, reg_bits):
             # per-workgroup partials grouped otherwise (the same terms)
             assert np.allclose(p1s[0], p1s[1], rtol=1e-13, atol=1e-300)
             got_on, got_off = on.get(), off.get()
-            assert np.array_equal(got_on, got_off)
             if fuse == 1:
+                assert np.array_equal(got_on, got_off)
                 assert np.array_equal(got_on, ref)
             else:
+                # fusion 2 reorders and contracts within a flush, and the two
+                # states may flush at different points (see below)
+                assert np.max(np.abs(got_on - got_off)) <= 1e-12
                 assert np.max(np.abs(got_on - ref)) <= 1e-12
         amps = [st.timed_amplitudes() for st in (on, off)]
         sweeps = [st.stats()[0] - s0 for st, s0 in zip((on, off), sw0)]
+        # (the sweep counts may differ: p1 of a qubit still 0 everywhere
+        # needs no flush, and with skipping off that knowledge ends at the
+        # first flush)
         assert amps[1] == sweeps[1] << n
-        assert sweeps[0] == sweeps[1]
         if n > 12:  # a 12-qubit state is one tile: nothing to skip
-            assert amps[0] < amps[1]
+            assert amps[0] < sweeps[0] << n
         else:
-            assert amps[0] == amps[1]
+            assert amps[0] == sweeps[0] << n
     finally:
         on.close()
         off.close()
@@ -174,7 +182,7 @@ def test_zero_tiles_complex64():
                 q = item[1]
                 p_ref = orc.prob_one(ref, n, q)
                 ps = [st.prob_one(q) for st in (on, off)]
-                assert abs(ps[0] - ps[1]) <= 1e-13 * ps[1] and abs(ps[0] - p_ref) <= 1e-5
+                assert abs(ps[0] - ps[1]) <= 1e-5 and abs(ps[0] - p_ref) <= 1e-5
                 outcome = 1 if p_ref > 0.5 else 0
                 norm = max(np.sqrt(p_ref), 1e-15) if outcome else max(np.sqrt(max(1 - p_ref, 0.0)), 1e-15)
                 for st in (on, off):
@@ -191,7 +199,7 @@ def test_zero_tiles_complex64():
                     st.apply_nq(u, qs)
                 orc.apply_n_qubit(ref, n, u, qs)
         got_on, got_off = on.get(), off.get()
-        assert np.array_equal(got_on, got_off)
+        assert np.max(np.abs(got_on - got_off)) <= 1e-5
         assert np.max(np.abs(got_on - ref)) <= 1e-5
     finally:
         on.close()
@@ -259,3 +267,33 @@ def test_random_blocks_engine_zero_tiles_equal():
     d = e1.compare(e0)
     assert d["max_abs"] == 0.0
     assert e1.timed_amplitudes() < e0.timed_amplitudes()
+
+
+def test_p1_of_an_untouched_qubit_reads_nothing():
+    """p1 of a qubit no queued or applied op has acted on other than
+    diagonally is +0.0 without a launch, and the queue stays unflushed; a
+    qubit a queued op acts on is reduced as usual."""
+    n = 16
+    st = capi.State(n)
+    try:
+        st.reset()
+        st.apply_1q(orc.gate_matrix("h", []), 0)
+        st.apply_1q(orc.gate_matrix("x", []), 15, [(0, 1)])   # cx 0 -> 15
+        st.apply_nq(orc.gate_matrix("cp", [0.4]), [3, 9])     # diagonal on untouched qubits
+        st.apply_1q(orc.gate_matrix("ry", [0.3]), 2, [(7, 1)])  # control on an untouched qubit
+        launches = st.stats()[1]
+        for q in (3, 7, 9, 14):
+            p = st.prob_one(q)
+            assert p == 0.0 and np.copysign(1.0, p) == 1.0
+        assert st.stats()[1] == launches
+        assert abs(st.prob_one(15) - 0.5) <= 1e-15
+        assert st.stats()[1] > launches
+        ref = np.zeros(1 << n, dtype=complex)
+        ref[0] = 1.0
+        orc.apply_one_qubit(ref, n, orc.gate_matrix("h", []), 0)
+        orc.apply_one_qubit(ref, n, orc.gate_matrix("x", []), 15, [(0, 1)])
+        orc.apply_n_qubit(ref, n, orc.gate_matrix("cp", [0.4]), [3, 9])
+        orc.apply_one_qubit(ref, n, orc.gate_matrix("ry", [0.3]), 2, [(7, 1)])
+        assert np.array_equal(st.get(), ref)
+    finally:
+        st.close()
