@@ -799,7 +799,7 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     uint64_t zero_q = s->zero_tiles ? s->zero_q : 0;
     for (size_t i = 0; i < passes.size(); ++i) {
         plan_pass(&cfg, s->queue, passes[i], mats, plans[i], zero_q);
-        zero_q &= ~pass_support(s->queue, passes[i]);
+        zero_q = s->zero_tiles ? zero_after_pass(s->queue, passes[i], zero_q) : 0;
         s->fused_blocks += uint64_t(plans[i].fused_blocks);
         s->fused_gates += uint64_t(plans[i].fused_gates);
         plans[i].pd.op_begin = int(total_ops);

@@ -838,6 +838,27 @@ inline uint64_t pass_support(const std::vector<HostOp> &queue, const Pass &p) {
     return m;
 }
 
+// Qubits a register pass leaves at 0 in every amplitude through a collapse
+// to outcome 0 (it writes exact zeros over the qubit's 1-half) that no later
+// op of the pass acts on other than diagonally.  Queue order is program
+// order, and the pass keeps the order of ops that do not commute.
+inline uint64_t pass_zeroed(const std::vector<HostOp> &queue, const Pass &p) {
+    std::vector<int> idx(p.ops);
+    std::sort(idx.begin(), idx.end());
+    uint64_t z = 0;
+    for (int i : idx) {
+        const HostOp &h = queue[size_t(i)];
+        if (h.kind == K_COLLAPSE && h.outcome == 0) z |= uint64_t(1) << h.q[0];
+        else z &= ~op_support(h);
+    }
+    return z;
+}
+
+// The qubits 0 everywhere after a register pass, from those before it.
+inline uint64_t zero_after_pass(const std::vector<HostOp> &queue, const Pass &p, uint64_t zero_q) {
+    return (zero_q & ~pass_support(queue, p)) | pass_zeroed(queue, p);
+}
+
 // Plans the layout of every op of one pass and encodes the ops.  zero_q:
 // qubits that are 0 in every amplitude before the pass.  Outer qubits among
 // them take the highest outer-index bits and are left out of pd.n_outer, so a

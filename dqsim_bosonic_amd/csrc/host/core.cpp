@@ -121,6 +121,13 @@ static std::vector<dq::HostOp> host_ops(py::list ops) {
         dq::HostOp op;
         op.k = int(qs.size());
         for (int j = 0; j < op.k; ++j) op.q[j] = qs[size_t(j)];
+        if (u.empty()) {  // ([qubit], [], [outcome]): a collapse
+            if (op.k != 1 || cv.size() != 1) throw std::invalid_argument("a collapse is ([qubit], [], [outcome])");
+            op.kind = dq::K_COLLAPSE;
+            op.outcome = cv[0];
+            queue.push_back(std::move(op));
+            continue;
+        }
         std::vector<double> flat(2 * u.size());
         for (size_t e = 0; e < u.size(); ++e) {
             flat[2 * e] = u[e].real();
@@ -500,7 +507,7 @@ PYBIND11_MODULE(_core, m) {
             dq::plan_pass(&cfg, queue, p, mats, rp, zero_q);
             std::vector<int> oq(rp.pd.oq, rp.pd.oq + (n - rp.b));
             out.append(py::make_tuple(p.S, p.ops, zero_q, rp.pd.n_outer, oq));
-            zero_q &= ~dq::pass_support(queue, p);
+            zero_q = dq::zero_after_pass(queue, p, zero_q);
         }
         return out;
     }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,

@@ -181,7 +181,7 @@ void check_plan(const dq::PlanConfig &cfg, const std::vector<dq::HostOp> &queue)
             }
             EXPECT(live == rp.pd.n_outer);
         }
-        zero_q &= ~dq::pass_support(queue, p);
+        zero_q = dq::zero_after_pass(queue, p, zero_q);
         const int R = rp.reg_bits;
         for (const dq::TileOp &t : rp.ops) {
             if (t.kind == dq::T_DENSE1 || t.kind == dq::T_CTRL1) EXPECT(t.p >= 0 && t.p < R);

@@ -281,39 +281,99 @@ def spreading_ops(n, count, rng):
     return ops
 
 
-@pytest.mark.parametrize("seed", range(4))
-@pytest.mark.parametrize("n,tile_bits,low_bits", [(14, 9, 4), (16, 10, 3)])
-def test_zero_qubits_before_each_pass(seed, n, tile_bits, low_bits):
+def apply_any(psi, n, qs, u, cv):
+    """apply_op, or a collapse ([q], [], [outcome]) as the engine applies it:
+    the other half set to exact zeros, the kept half divided by its norm."""
+    if len(u) == 0:
+        idx = np.arange(1 << n)
+        keep = ((idx >> qs[0]) & 1) == cv[0]
+        nrm = np.sqrt(np.sum(np.abs(psi[keep]) ** 2))
+        psi = psi.copy()
+        psi[~keep] = 0.0
+        if nrm > 0:
+            psi[keep] /= nrm
+        return psi
+    return apply_op(psi, n, qs, u, cv)
+
+
+def with_collapses(n, ops, rng, count):
+    """ops with `count` collapses inserted at random places, each on a qubit
+    that an earlier op acted on, to outcome 0 unless that has probability
+    below 1e-6 (so most of them write zeros over the qubit's 1-half)."""
+    at = set(int(x) for x in rng.choice(np.arange(10, len(ops)), size=count, replace=False))
+    out = []
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[0] = 1.0
+    idx = np.arange(1 << n)
+    for i, op in enumerate(ops):
+        if i in at:
+            q = int(rng.choice(sorted({q for o in out if len(o[1]) for q in o[0]})))
+            p0 = float(np.sum(np.abs(psi[((idx >> q) & 1) == 0]) ** 2))
+            c = ([q], [], [0 if p0 > 1e-6 else 1])
+            out.append(c)
+            psi = apply_any(psi, n, *c)
+        out.append(op)
+        psi = apply_any(psi, n, *op)
+    return out
+
+
+def check_zero_plan(n, ops, tile_bits, low_bits):
     """Applied to |0...0>, the qubits a flush takes as 0 before a pass are 0
     in every amplitude there (exact zeros in the oracle's statevector), the
     outer index puts them after the others (ascending within each group) and
-    n_outer counts the others; the first pass visits a single tile."""
-    rng = np.random.default_rng(300 + seed)
-    ops = spreading_ops(n, 120, rng)
+    n_outer counts the others.  Returns (plan, outer qubits skipped over all
+    passes, qubits taken as 0 again after an op acted on them)."""
     plan = core._zero_passes(n, [(qs, np.asarray(u).ravel(), cv) for qs, u, cv in ops], 2, tile_bits, low_bits)
-    assert plan[0][3] == 0
     idx = np.arange(1 << n)
     psi = np.zeros(1 << n, dtype=complex)
     psi[0] = 1.0
-    skipped = 0
-    prev = (1 << n) - 1
+    skipped = regained = 0
+    touched = 0
     for S, pass_ops, zero_q, n_outer, oq in plan:
-        assert zero_q & ~prev == 0  # only shrinks
-        prev = zero_q
+        regained += bin(zero_q & touched).count("1")
         outer = [q for q in range(n) if not (S >> q) & 1]
         live = [q for q in outer if not (zero_q >> q) & 1]
         assert oq == live + [q for q in outer if (zero_q >> q) & 1]
         assert n_outer == len(live)
         assert np.all(psi[(idx & zero_q) != 0] == 0)
         skipped += len(outer) - n_outer
-        for i in pass_ops:
-            psi = apply_op(psi, n, *ops[i])
-    assert skipped > 0
+        for i in sorted(pass_ops):
+            psi = apply_any(psi, n, *ops[i])
+        touched |= S
     want = np.zeros(1 << n, dtype=complex)
     want[0] = 1.0
     for op in ops:
-        want = apply_op(want, n, *op)
+        want = apply_any(want, n, *op)
     assert np.max(np.abs(psi - want)) < 1e-12
+    return plan, skipped, regained
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("n,tile_bits,low_bits", [(14, 9, 4), (16, 10, 3)])
+def test_zero_qubits_before_each_pass(seed, n, tile_bits, low_bits):
+    """The check above on programs whose qubits spread upward; without
+    collapses the set only shrinks, and the first pass visits one tile."""
+    rng = np.random.default_rng(300 + seed)
+    ops = spreading_ops(n, 120, rng)
+    plan, skipped, _ = check_zero_plan(n, ops, tile_bits, low_bits)
+    assert plan[0][3] == 0
+    assert skipped > 0
+    prev = (1 << n) - 1
+    for _, _, zero_q, _, _ in plan:
+        assert zero_q & ~prev == 0
+        prev = zero_q
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("n,tile_bits,low_bits", [(14, 9, 4), (16, 10, 3)])
+def test_zero_qubits_with_collapses(seed, n, tile_bits, low_bits):
+    """With collapses between the ops: a qubit collapsed to 0, with no later
+    op of its pass acting on it other than diagonally, is taken as 0 again
+    from the next pass, and the check above holds."""
+    rng = np.random.default_rng(400 + seed)
+    ops = with_collapses(n, spreading_ops(n, 120, rng), rng, 24)
+    _, _, regained = check_zero_plan(n, ops, tile_bits, low_bits)
+    assert regained > 0
 
 
 def test_zero_qubit_invariant_catches_a_wrong_support():
