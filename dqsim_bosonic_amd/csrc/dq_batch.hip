@@ -633,7 +633,8 @@ PersistPlan plan_persist(const BInstance &in, unsigned elem) {
     }
     // the slot starts as |0...0>: qubits no pass has touched yet are 0
     // everywhere, and each pass visits only the tiles where they are 0
-    uint64_t zero_q = ~uint64_t(0) >> (64 - in.n);
+    KnownBits kb;
+    kb.zero = ~uint64_t(0) >> (64 - in.n);
     for (size_t c0 = 0; c0 < all.size(); c0 += 4096) {
         std::vector<HostOp> queue(all.begin() + long(c0), all.begin() + long(std::min(all.size(), c0 + 4096)));
         for (const Pass &p : schedule(&cfg, queue, true)) {
@@ -642,8 +643,8 @@ PersistPlan plan_persist(const BInstance &in, unsigned elem) {
                 return pp;
             }
             RegPlan rp;
-            plan_pass(&cfg, queue, p, pp.mats, rp, zero_q);
-            zero_q &= ~pass_support(queue, p);
+            plan_pass(&cfg, queue, p, pp.mats, rp, kb.zero);
+            advance_known(queue, p, kb);
             pp.swept_amps += std::ldexp(1.0, rp.b + rp.pd.n_outer);
             if (rp.b != kPersistTileBits) {
                 pp.ok = false;

@@ -796,10 +796,11 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     size_t total_ops = 0;
     // qubits still 0 everywhere before each pass; a pass's sweep skips the
     // tiles where any outer one of them is 1 (they hold zeros and stay zero)
-    uint64_t zero_q = s->zero_tiles ? s->zero_q : 0;
+    KnownBits kb;
+    kb.zero = s->zero_tiles ? s->zero_q : 0;
     for (size_t i = 0; i < passes.size(); ++i) {
-        plan_pass(&cfg, s->queue, passes[i], mats, plans[i], zero_q);
-        zero_q = s->zero_tiles ? zero_after_pass(s->queue, passes[i], zero_q) : 0;
+        plan_pass(&cfg, s->queue, passes[i], mats, plans[i], kb.zero);
+        if (s->zero_tiles) advance_known(s->queue, passes[i], kb);
         s->fused_blocks += uint64_t(plans[i].fused_blocks);
         s->fused_gates += uint64_t(plans[i].fused_gates);
         plans[i].pd.op_begin = int(total_ops);
@@ -844,7 +845,7 @@ int flush_reg(dq_state *s, const std::vector<Pass> &passes, int red_qubit = -1) 
     s->zero_q = 0;  // unknown should a launch fail part way
     for (const RegPlan &rp : plans)
         if (int rc = launch_reg<T>(s, rp, d_ops, d_mats)) return rc;
-    s->zero_q = zero_q;
+    s->zero_q = kb.zero;
     s->queue.clear();
     return DQ_OK;
 }

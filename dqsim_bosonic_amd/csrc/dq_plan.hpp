@@ -830,33 +830,44 @@ inline uint64_t op_support(const HostOp &h) {
     return h.qmask();
 }
 
-// Qubits a register pass can leave nonzero: its tile qubits and the support
-// of each of its ops.
-inline uint64_t pass_support(const std::vector<HostOp> &queue, const Pass &p) {
-    uint64_t m = p.S;
-    for (int i : p.ops) m |= op_support(queue[size_t(i)]);
-    return m;
+// Qubits known to be 0 (zero) or 1 (one) in every amplitude.
+struct KnownBits {
+    uint64_t zero = 0, one = 0;
+};
+
+// Whether a queued op is a 2x2 with exact zeros on its diagonal (x, y, any
+// anti-diagonal) on one qubit: it maps a qubit that is 1 everywhere to one
+// that is 0 everywhere and back, each new amplitude an exact product with
+// one old amplitude plus an exact zero product.
+inline bool is_antidiag1(const HostOp &h) {
+    if (h.k != 1 || h.mat.size() != 8) return false;
+    if (!(h.kind == K_DENSE1 || (h.kind == K_CTRL1 && h.cmask == 0))) return false;
+    return is_zero(h.mat.data()) && is_zero(h.mat.data() + 6);
 }
 
-// Qubits a register pass leaves at 0 in every amplitude through a collapse
-// to outcome 0 (it writes exact zeros over the qubit's 1-half) that no later
-// op of the pass acts on other than diagonally.  Queue order is program
-// order, and the pass keeps the order of ops that do not commute.
-inline uint64_t pass_zeroed(const std::vector<HostOp> &queue, const Pass &p) {
+// Advances the known bits over one register pass, its ops in queue order
+// (program order; a pass keeps the order of ops that do not commute): a
+// collapse makes its qubit known (it writes exact zeros over the other
+// half), an anti-diagonal 2x2 swaps 0 and 1, diagonal ops and controls keep
+// what is known, and any other op forgets its support.
+inline void advance_known(const std::vector<HostOp> &queue, const Pass &p, KnownBits &kb) {
     std::vector<int> idx(p.ops);
     std::sort(idx.begin(), idx.end());
-    uint64_t z = 0;
     for (int i : idx) {
         const HostOp &h = queue[size_t(i)];
-        if (h.kind == K_COLLAPSE && h.outcome == 0) z |= uint64_t(1) << h.q[0];
-        else z &= ~op_support(h);
+        const uint64_t b = uint64_t(1) << h.q[h.k - 1];
+        if (h.kind == K_COLLAPSE) {
+            (h.outcome ? kb.one : kb.zero) |= b;
+            (h.outcome ? kb.zero : kb.one) &= ~b;
+        } else if (is_antidiag1(h)) {
+            const bool z = (kb.zero & b) != 0, o = (kb.one & b) != 0;
+            kb.zero = (kb.zero & ~b) | (o ? b : 0);
+            kb.one = (kb.one & ~b) | (z ? b : 0);
+        } else {
+            kb.zero &= ~op_support(h);
+            kb.one &= ~op_support(h);
+        }
     }
-    return z;
-}
-
-// The qubits 0 everywhere after a register pass, from those before it.
-inline uint64_t zero_after_pass(const std::vector<HostOp> &queue, const Pass &p, uint64_t zero_q) {
-    return (zero_q & ~pass_support(queue, p)) | pass_zeroed(queue, p);
 }
 
 // Plans the layout of every op of one pass and encodes the ops.  zero_q:

@@ -499,15 +499,16 @@ PYBIND11_MODULE(_core, m) {
         cfg.low_bits = low_bits;
         cfg.reg_bits = reg_bits;
         if (!(fuse && dq::effective_tile_bits(&cfg) >= 9)) throw std::invalid_argument("not a register-tile plan");
-        uint64_t zero_q = n ? ~uint64_t(0) >> (64 - n) : 0;
+        dq::KnownBits kb;
+        kb.zero = n ? ~uint64_t(0) >> (64 - n) : 0;
         py::list out;
         std::vector<double> mats;
         for (const dq::Pass &p : dq::schedule(&cfg, queue, true)) {
             dq::RegPlan rp;
-            dq::plan_pass(&cfg, queue, p, mats, rp, zero_q);
+            dq::plan_pass(&cfg, queue, p, mats, rp, kb.zero);
             std::vector<int> oq(rp.pd.oq, rp.pd.oq + (n - rp.b));
-            out.append(py::make_tuple(p.S, p.ops, zero_q, rp.pd.n_outer, oq));
-            zero_q = dq::zero_after_pass(queue, p, zero_q);
+            out.append(py::make_tuple(p.S, p.ops, kb.zero, rp.pd.n_outer, oq));
+            dq::advance_known(queue, p, kb);
         }
         return out;
     }, py::arg("n"), py::arg("ops"), py::arg("fuse") = 2, py::arg("tile_bits") = 12, py::arg("low_bits") = 4,

@@ -156,7 +156,9 @@ void check_plan(const dq::PlanConfig &cfg, const std::vector<dq::HostOp> &queue)
     std::vector<int> seen(queue.size(), 0);
     std::vector<double> mats;
     // qubits 0 everywhere before each pass, as a flush from |0...0> tracks them
-    uint64_t zero_q = cfg.n ? ~uint64_t(0) >> (64 - cfg.n) : 0;
+    dq::KnownBits kb;
+    kb.zero = cfg.n ? ~uint64_t(0) >> (64 - cfg.n) : 0;
+    const uint64_t &zero_q = kb.zero;
     for (const dq::Pass &p : passes) {
         for (int i : p.ops) {
             EXPECT(i >= 0 && size_t(i) < queue.size());
@@ -165,7 +167,7 @@ void check_plan(const dq::PlanConfig &cfg, const std::vector<dq::HostOp> &queue)
         if (!p.S || !anywhere) continue;
         EXPECT(__builtin_popcountll(p.S) == dq::effective_tile_bits(&cfg));
         dq::RegPlan rp;
-        dq::plan_pass(&cfg, queue, p, mats, rp, zero_q);
+        dq::plan_pass(&cfg, queue, p, mats, rp, kb.zero);
         {
             // the outer index lists every outer qubit once, the ones not known
             // to be 0 first; n_outer counts those
@@ -181,7 +183,7 @@ void check_plan(const dq::PlanConfig &cfg, const std::vector<dq::HostOp> &queue)
             }
             EXPECT(live == rp.pd.n_outer);
         }
-        zero_q = dq::zero_after_pass(queue, p, zero_q);
+        dq::advance_known(queue, p, kb);
         const int R = rp.reg_bits;
         for (const dq::TileOp &t : rp.ops) {
             if (t.kind == dq::T_DENSE1 || t.kind == dq::T_CTRL1) EXPECT(t.p >= 0 && t.p < R);

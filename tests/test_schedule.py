@@ -298,8 +298,9 @@ def apply_any(psi, n, qs, u, cv):
 
 def with_collapses(n, ops, rng, count):
     """ops with `count` collapses inserted at random places, each on a qubit
-    that an earlier op acted on, to outcome 0 unless that has probability
-    below 1e-6 (so most of them write zeros over the qubit's 1-half)."""
+    that an earlier op acted on, to outcome 0 or 1 (one with probability
+    above 1e-6), a collapse to 1 followed by an x or a y on the qubit (as a
+    reset's)."""
     at = set(int(x) for x in rng.choice(np.arange(10, len(ops)), size=count, replace=False))
     out = []
     psi = np.zeros(1 << n, dtype=complex)
@@ -309,9 +310,14 @@ def with_collapses(n, ops, rng, count):
         if i in at:
             q = int(rng.choice(sorted({q for o in out if len(o[1]) for q in o[0]})))
             p0 = float(np.sum(np.abs(psi[((idx >> q) & 1) == 0]) ** 2))
-            c = ([q], [], [0 if p0 > 1e-6 else 1])
+            p1 = float(np.sum(np.abs(psi[((idx >> q) & 1) == 1]) ** 2))
+            c = ([q], [], [0 if p0 > 1e-6 and (p1 <= 1e-6 or rng.random() < 0.5) else 1])
             out.append(c)
             psi = apply_any(psi, n, *c)
+            if c[2][0] == 1:  # a reset's x, or a y: the qubit is 0 everywhere after it
+                x = ([q], orc.gate_matrix("x" if rng.random() < 0.7 else "y", []), [])
+                out.append(x)
+                psi = apply_any(psi, n, *x)
         out.append(op)
         psi = apply_any(psi, n, *op)
     return out
@@ -367,9 +373,10 @@ def test_zero_qubits_before_each_pass(seed, n, tile_bits, low_bits):
 @pytest.mark.parametrize("seed", range(4))
 @pytest.mark.parametrize("n,tile_bits,low_bits", [(14, 9, 4), (16, 10, 3)])
 def test_zero_qubits_with_collapses(seed, n, tile_bits, low_bits):
-    """With collapses between the ops: a qubit collapsed to 0, with no later
-    op of its pass acting on it other than diagonally, is taken as 0 again
-    from the next pass, and the check above holds."""
+    """With collapses between the ops: a qubit collapsed to 0, or collapsed
+    to 1 and then flipped by an anti-diagonal 2x2, with no later op acting on
+    it other than diagonally, is taken as 0 again, and the check above
+    holds."""
     rng = np.random.default_rng(400 + seed)
     ops = with_collapses(n, spreading_ops(n, 120, rng), rng, 24)
     _, _, regained = check_zero_plan(n, ops, tile_bits, low_bits)
