@@ -1094,7 +1094,7 @@ int dq_state_reset(dq_state *s) {
                            static_cast<cplx<float> *>(s->amps), count);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
-    s->zero_q = s->n ? ~uint64_t(0) >> (64 - s->n) : 0;
+    s->zero_q = s->n && s->zero_tiles ? ~uint64_t(0) >> (64 - s->n) : 0;
     return DQ_OK;
 }
 
@@ -1190,7 +1190,7 @@ int dq_prob_one(dq_state *s, int qubit, double *p1) {
     // a qubit 0 in every amplitude in memory, on which no queued op acts
     // other than diagonally, is still 0 everywhere: the sum over its 1-half
     // is a sum of exact zeros, +0.0, with nothing flushed or read
-    if ((s->zero_q >> qubit) & 1) {
+    if (s->zero_tiles && ((s->zero_q >> qubit) & 1)) {
         bool touched = false;
         for (const HostOp &h : s->queue) touched = touched || ((op_support(h) >> qubit) & 1);
         if (!touched) {
@@ -1925,6 +1925,7 @@ int dq_set_zero_tiles(dq_state *s, int on) {
     if (!s) return fail(DQ_ERR_ARG, "null state");
     if (int rc = flush(s)) return rc;
     s->zero_tiles = on ? 1 : 0;
+    if (!on) s->zero_q = 0;  // known again from the next reset
     return DQ_OK;
 }
 

@@ -185,10 +185,12 @@ int dq_get_timing(dq_state *s, double *sweep_ms, uint64_t *sweep_launches);
  * than sweeps x 2^n. */
 int dq_get_timed_amplitudes(dq_state *s, uint64_t *amps);
 
-/* Whether register sweeps skip the tiles where a qubit still 0 in every
- * amplitude is 1 (default on; no reference counterpart).  Those tiles hold
- * zeros that a sweep would leave zero, so either setting gives the same
- * state; off is for timing comparisons. */
+/* Whether the engine uses the qubits it knows to be 0 in every amplitude
+ * (default on; no reference counterpart): register sweeps skip the tiles
+ * where such a qubit is 1, and dq_prob_one of such a qubit returns +0.0
+ * without reading the state.  Those tiles hold zeros that a sweep would
+ * leave zero, so either setting gives the same state; off (until the next
+ * reset after switching on again) is for timing comparisons. */
 int dq_set_zero_tiles(dq_state *s, int on);
 
 /* Number of HBM sweeps and kernel launches issued so far on this handle. */

@@ -152,8 +152,7 @@ def test_zero_tiles_equal_full_sweeps_and_oracle(n, fuse, reg_bits):
         amps = [st.timed_amplitudes() for st in (on, off)]
         sweeps = [st.stats()[0] - s0 for st, s0 in zip((on, off), sw0)]
         # (the sweep counts may differ: p1 of a qubit still 0 everywhere
-        # needs no flush, and with skipping off that knowledge ends at the
-        # first flush)
+        # needs no flush with skipping on)
         assert amps[1] == sweeps[1] << n
         if n > 12:  # a 12-qubit state is one tile: nothing to skip
             assert amps[0] < sweeps[0] << n
